@@ -1,0 +1,238 @@
+/*
+ * tsn.h - C ABI of the MI355X (gfx950) fixed-trajectory NUFFT / CG-SENSE path.
+ *
+ * Plain C: pointers, sizes and status codes only.  Device pointers are HIP
+ * device addresses; `stream` is a hipStream_t passed as void* (0 = null
+ * stream).  Every entry point returns TSN_OK or a status code mirroring the
+ * reference's process exit codes (nufft_spmma_bench.cu:61-69, 92-96;
+ * nufft_integrated_bench.cu:141-145, 189-194; nufft_fp16x2_cg_bench.cu:
+ * 317-322, 418-421).  The library never calls exit(); tsn_last_error()
+ * returns the message of the last failure on the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference repo):
+ *   pack directory format  src/export_torchkbnufft_gpu_case.py:387-436 (writer)
+ *                          experiments/production/src/nufft_integrated_bench.cu:135-226 (reader)
+ *   FFT endpoint ABI       experiments/production/src/nufft_custom_fft_f30_api.h:6-34
+ *   operator / solver      nufft_integrated_bench.cu:431-466 (launch_fused_interpolation[_residual]),
+ *                          nufft_fp16x2_cg_bench.cu:203-268 (launch_adjoint_image, launch_normal, launch_cg)
+ */
+#ifndef TSN_H
+#define TSN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. 1/2/3 carry the reference binaries' exit-code meaning. */
+enum {
+  TSN_OK = 0,
+  TSN_ERR_RUNTIME = 1,      /* HIP error, unreadable/short file, bad header (ref exit 1) */
+  TSN_ERR_INCOMPATIBLE = 2, /* real/imag or operator shape mismatch (ref exit 2) */
+  TSN_ERR_CG = 3,           /* CG non-positive curvature or non-finite output (ref exit 3) */
+  TSN_ERR_ARGUMENT = 4      /* invalid argument or unsupported configuration */
+};
+
+/* Interpolation precision / execution variants. */
+enum {
+  TSN_PREC_SPLIT16 = 0,  /* FP16 hi + FP16 residual panels, 2:4 sparse MFMA, FP32 accumulate */
+  TSN_PREC_DENSE16 = 1,  /* same panels and tiles on dense MFMA (equal-quality control) */
+  TSN_PREC_FP32 = 2,     /* plain FP32 CSR gather with FP32 coefficients */
+  TSN_PREC_HI16 = 3,     /* FP16 hi panel only on 2:4 sparse MFMA */
+  TSN_PREC_SPLITBF16 = 4 /* BF16 hi + BF16 residual panels, BF16 tiles, 2:4 sparse MFMA */
+};
+
+/* Output layouts of the interpolation entry points. */
+enum {
+  TSN_OUT_COIL_MAJOR = 0, /* float2 [coil][row]   (reference output layout) */
+  TSN_OUT_ROW_MAJOR = 1   /* float2 [row][coil]   (coil-interleaved rows)  */
+};
+
+/* Pack header, 52 bytes little endian, struct "<Q11I"
+ * (reference src/export_gpu_case.py:242-256, nufft_spmma_bench.cu:42-59). */
+#pragma pack(push, 1)
+typedef struct tsn_pack_header {
+  uint64_t magic; /* 0x4E55464654323431 "NUFFT241" */
+  uint32_t version; /* 2 */
+  uint32_t n_channels; /* 16 */
+  uint32_t output_size;
+  uint32_t n_cols;
+  uint32_t active_rows;
+  uint32_t nnz;
+  uint32_t groups;
+  uint32_t tiles;
+  uint32_t residual_nnz; /* must be 0 */
+  uint32_t m_tile; /* 16 */
+  uint32_t k_tile; /* 32 */
+} tsn_pack_header;
+#pragma pack(pop)
+
+const char* tsn_last_error(void);
+/* Library build identification (for logs). */
+const char* tsn_version(void);
+
+/* ------------------------------------------------------------------ packs */
+
+/* One packed direction (real + imag components sharing one structure), in
+ * host memory.  Replaces the reference's per-component pack directories. */
+typedef struct tsn_host_pack tsn_host_pack;
+
+/* Read and validate a real/imag directory pair (reference load_direction,
+ * nufft_integrated_bench.cu:135-194: same header checks, identical structure,
+ * residual_nnz == 0) plus bounds checks of every index array. */
+int tsn_host_pack_read(const char* real_dir, const char* imag_dir, tsn_host_pack** out);
+/* Write the pair in the reference directory format
+ * (export_torchkbnufft_gpu_case.py:407-436; production-only file set). */
+int tsn_host_pack_write(const tsn_host_pack* pack, const char* real_dir, const char* imag_dir);
+/* Build a pack from caller arrays (copied and validated). */
+int tsn_host_pack_create(const tsn_pack_header* header, const uint32_t* group_offsets,
+                         const int32_t* group_row_ids, const int32_t* tile_col_ids,
+                         const uint16_t* real_comp, const uint32_t* real_meta,
+                         const uint16_t* imag_comp, const uint32_t* imag_meta,
+                         tsn_host_pack** out);
+int tsn_host_pack_header(const tsn_host_pack* pack, tsn_pack_header* out);
+/* Borrowed pointers valid until tsn_host_pack_free. */
+int tsn_host_pack_arrays(const tsn_host_pack* pack, const uint32_t** group_offsets,
+                         const int32_t** group_row_ids, const int32_t** tile_col_ids,
+                         const uint16_t** real_comp, const uint32_t** real_meta,
+                         const uint16_t** imag_comp, const uint32_t** imag_meta);
+void tsn_host_pack_free(tsn_host_pack* pack);
+
+/* ------------------------------------------------------------- KB matrix */
+
+/* Complex KB interpolation matrix G (samples x grid), float32 components,
+ * numpoints^2 entries per row, row-sorted columns.  Follows torchkbnufft
+ * calc_tensor_spmatrix as called at export_torchkbnufft_gpu_case.py:537-542.
+ * traj: [samples][2] float32 in cycles/pixel, [-0.5, 0.5); traj[m][0] is the
+ * slow image axis.  threads <= 0 uses all hardware threads. */
+typedef struct tsn_kb_matrix tsn_kb_matrix;
+int tsn_kb_build(const float* traj, uint32_t samples, uint32_t image0, uint32_t image1,
+                 uint32_t grid0, uint32_t grid1, uint32_t numpoints, int threads,
+                 tsn_kb_matrix** out);
+int tsn_kb_arrays(const tsn_kb_matrix* kb, uint32_t* rows, uint32_t* width, const int32_t** cols,
+                  const float** real, const float** imag);
+/* Apodization (torchkbnufft scaling_coef, real): float32 [image0][image1]. */
+int tsn_kb_apodization(uint32_t image0, uint32_t image1, uint32_t grid0, uint32_t grid1,
+                       uint32_t numpoints, float* out);
+/* Pack view 0 (G, rows = samples) or 1 (G_T, rows = active grid points)
+ * with the production planner (policy 0 = morton, 1 = trajectory order),
+ * the reference default seed being 20260823. */
+int tsn_kb_pack(const tsn_kb_matrix* kb, int view, int policy, uint64_t seed, int threads,
+                tsn_host_pack** out);
+void tsn_kb_free(tsn_kb_matrix* kb);
+
+/* ------------------------------------------------------- device directions */
+
+typedef struct tsn_direction tsn_direction;
+/* Upload a pack to the current device (tiles re-laid out per MFMA lane). */
+int tsn_direction_upload(const tsn_host_pack* pack, tsn_direction** out);
+/* Reference load_direction: read + validate + upload. */
+int tsn_direction_load(const char* real_dir, const char* imag_dir, tsn_direction** out);
+/* FP32 CSR direction of the plain path: view 0 = G, 1 = G_T (exact transpose). */
+int tsn_direction_upload_csr(const tsn_kb_matrix* kb, int view, tsn_direction** out);
+int tsn_direction_header(const tsn_direction* dir, tsn_pack_header* out);
+/* Device bytes the direction occupies (pack + index arrays). */
+size_t tsn_direction_bytes(const tsn_direction* dir);
+void tsn_direction_free(tsn_direction* dir);
+
+/* Interpolation y = G x (conjugate = 0) or G^H-style conjugate combine (1)
+ * of FP16 N16 panels ([row][16] halves: Re coils 0..7, Im coils 0..7, blocks
+ * of 8 coils, block stride n_cols*16 halves).  panel_res may be NULL (hi
+ * only).  Output float2 per layout; accumulate adds to it.  scale multiplies
+ * the result.  Equivalent to launch_fused_interpolation followed by
+ * launch_fused_interpolation_residual (nufft_integrated_bench.cu:431-466). */
+int tsn_interp(const tsn_direction* dir, const void* panel_hi, const void* panel_res, void* out,
+               uint32_t coils, int conjugate, int accumulate, float scale, int out_layout,
+               void* stream);
+/* Same operator on the dense-MFMA control (tiles expanded to 16x32 at upload). */
+int tsn_interp_dense(const tsn_direction* dir, const void* panel_hi, const void* panel_res,
+                     void* out, uint32_t coils, int conjugate, int accumulate, float scale,
+                     int out_layout, void* stream);
+/* Plain FP32 path: CSR direction, input float2 [row][coils], output per layout. */
+int tsn_interp_f32(const tsn_direction* csr, const void* in, void* out, uint32_t coils,
+                   int conjugate, int accumulate, float scale, int out_layout, void* stream);
+/* hi = RN16(v), res = RN16(v - hi) of float2 [coil][rows] * scale into N16 panel
+ * blocks (reference integrated_complex_to_split2_half_rows, :339-359). */
+int tsn_split_panel(const void* in, uint32_t rows, uint32_t coils, float scale, void* hi, void* res,
+                    void* stream);
+
+/* ------------------------------------------------------------------ plan */
+
+typedef struct tsn_plan_config {
+  uint32_t image0, image1; /* image shape (n0 slow) */
+  uint32_t grid0, grid1;   /* oversampled grid = 2*image, sides in {64,128,256,512,768} */
+  uint32_t coils;          /* 1..16 */
+  int precision;           /* TSN_PREC_* */
+} tsn_plan_config;
+
+typedef struct tsn_plan tsn_plan;
+/* forward: G pack (rows = samples), adjoint: G_T pack (rows = grid points);
+ * both CSR directions for TSN_PREC_FP32.  smaps: device float2 [coils][image0][image1];
+ * apodization: device float [image0][image1] (torchkbnufft scaling_coef).
+ * The plan copies what it needs; all work buffers are allocated here. */
+int tsn_plan_create(const tsn_plan_config* config, const tsn_direction* forward,
+                    const tsn_direction* adjoint, const void* smaps, const void* apodization,
+                    tsn_plan** out);
+void tsn_plan_free(tsn_plan* plan);
+size_t tsn_plan_bytes(const tsn_plan* plan);
+
+/* A x: image float2 [image0][image1] -> samples float2 [coils][M], normalised by
+ * 1/sqrt(grid0*grid1) (the native scaling of nufft_fp16x2_cg_bench.cu:230-236). */
+int tsn_forward(tsn_plan* plan, const void* image, void* samples, void* stream);
+/* A^H y (exact adjoint of tsn_forward). */
+int tsn_adjoint(tsn_plan* plan, const void* samples, void* image, void* stream);
+/* (A^H A) x * inv_scale_sq + scaled_lambda * x (launch_normal). */
+int tsn_normal(tsn_plan* plan, const void* image, void* out, float inv_scale_sq,
+               float scaled_lambda, void* stream);
+
+/* CG-SENSE (launch_cg): measurement raw float2 [coils][M] (torchkbnufft scale),
+ * operator_scale as the reference CLI argument (native scale), lambda the raw
+ * Tikhonov weight (reference 1e-4).  Writes x float2 [image0][image1], the
+ * residual history sqrt(rho) as double[iterations+1] and a status int
+ * (1 = ok, 0 = non-positive curvature) to device memory, asynchronously.
+ * CG scalars (rho, <p,Ap>, alpha, beta) are float64 on the device. */
+int tsn_cg(tsn_plan* plan, const void* measurement, uint32_t iterations, float operator_scale,
+           float lambda, void* x_out, void* residuals_out, void* status_out, void* stream);
+/* Capture the same solve into a hipGraph bound to these pointers; launch it
+ * any number of times; results are bitwise equal to tsn_cg. */
+int tsn_cg_graph_capture(tsn_plan* plan, const void* measurement, uint32_t iterations,
+                         float operator_scale, float lambda, void* x_out, void* residuals_out,
+                         void* status_out, void* stream);
+int tsn_cg_graph_launch(tsn_plan* plan, void* stream);
+
+/* ----------------------------------------------- FFT endpoint drop-in ABI */
+/* Same names and argument meaning as nufft_custom_fft_f30_api.h:6-34 (fixed
+ * 256^2 image, 512^2 grid, 8 coils).  Pointers: image float2, scaling float,
+ * transposed_workspace float2 [8][512][512] (pre-zeroed), N16 outputs __half
+ * [512*512][16], stream hipStream_t.  Returns hipError_t as int. */
+int jkf_initialize_custom_fft_twiddles(void);
+void jkf_launch_custom_fft_f30_forward(const void* image, const float* scaling,
+                                       void* transposed_workspace, void* n16_output, void* stream);
+void jkf_launch_custom_fft_f30_forward_split2(const void* image, const float* scaling,
+                                              void* transposed_workspace, void* n16_high_output,
+                                              void* n16_residual_output, void* stream);
+void jkf_launch_custom_fft_f30_forward_split2_debug(const void* image, const float* scaling,
+                                                    void* transposed_workspace,
+                                                    void* n16_high_output,
+                                                    void* n16_residual_output,
+                                                    void* fp32_endpoint_output, void* stream);
+void jkf_launch_custom_fft_f30_forward_sense(const void* image, const void* sensitivity_maps,
+                                             const float* scaling, void* transposed_workspace,
+                                             void* n16_output, void* stream);
+void jkf_launch_custom_fft_f30_forward_sense_split2(const void* image,
+                                                    const void* sensitivity_maps,
+                                                    const float* scaling,
+                                                    void* transposed_workspace,
+                                                    void* n16_high_output,
+                                                    void* n16_residual_output, void* stream);
+void jkf_launch_custom_ifft_f31_adjoint(const void* grid_input, void* transposed_workspace,
+                                        const float* scaling, const float* scaling_tiled,
+                                        void* image_output, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TSN_H */

@@ -1,0 +1,148 @@
+"""GPU parity of the interpolation kernels against the float64 oracle.
+
+Reference gates: native vs arithmetic model <= 2e-5 relative
+(``validate_fp16x2_native.py:251-252``); split panels bitwise equal to numpy
+FP16 round-to-nearest (``:175-216``); fused vs unfused <= 2e-6
+(``nufft_integrated_bench.cu:1347-1353``).
+"""
+
+import numpy as np
+import pytest
+
+from gpu_util import dev, need_gpu, random_panels, rel_l2, small_case
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+TOL = 2e-5
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    need_gpu()
+
+
+def _direction(view):
+    from trajsparsenufft_amd.device import Direction
+    _, packs = small_case()
+    pack = packs.forward if view == "G" else packs.adjoint
+    return Direction.upload(pack), pack.arrays(), pack.header
+
+
+@pytest.mark.parametrize("view,conj", [("G", False), ("G_T", True), ("G", True), ("G_T", False)])
+@pytest.mark.parametrize("coils", [8, 4, 12, 16])
+def test_sparse_split_matches_oracle(view, conj, coils):
+    from oracle.solver import apply_pack
+    from trajsparsenufft_amd import device
+    d, arrays, header = _direction(view)
+    _, hi, res = random_panels(header["n_cols"], coils, seed=coils)
+    out = torch.zeros((coils, header["output_size"]), dtype=torch.complex64, device="cuda")
+    device.interp(d, dev(hi), dev(res), out, coils=coils, conjugate=conj, scale=0.5)
+    ref = apply_pack(arrays, header["output_size"], hi, res, coils, conj, 0.5)
+    assert rel_l2(out.cpu().numpy(), ref) <= TOL
+
+
+@pytest.mark.parametrize("view", ["G", "G_T"])
+def test_layouts_accumulate_dense_and_hi_only(view):
+    from oracle.solver import apply_pack
+    from trajsparsenufft_amd import device
+    coils = 8
+    d, arrays, header = _direction(view)
+    n_out = header["output_size"]
+    _, hi, res = random_panels(header["n_cols"], coils, seed=5)
+    ref = apply_pack(arrays, n_out, hi, res, coils, view == "G_T")
+    conj = view == "G_T"
+    # row-major output
+    out_rm = torch.zeros((n_out, coils), dtype=torch.complex64, device="cuda")
+    device.interp(d, dev(hi), dev(res), out_rm, coils=coils, conjugate=conj, layout="row_major")
+    assert rel_l2(out_rm.cpu().numpy().T, ref) <= TOL
+    # reference two-pass semantics: hi pass stores, residual pass accumulates
+    out = torch.zeros((coils, n_out), dtype=torch.complex64, device="cuda")
+    device.interp(d, dev(hi), None, out, coils=coils, conjugate=conj)
+    ref_hi = apply_pack(arrays, n_out, hi, None, coils, conj)
+    assert rel_l2(out.cpu().numpy(), ref_hi) <= TOL
+    device.interp(d, dev(res), None, out, coils=coils, conjugate=conj, accumulate=True)
+    assert rel_l2(out.cpu().numpy(), ref) <= TOL
+    # dense-MFMA control on the same tiles
+    out_d = torch.zeros((coils, n_out), dtype=torch.complex64, device="cuda")
+    device.interp(d, dev(hi), dev(res), out_d, coils=coils, conjugate=conj, dense=True)
+    assert rel_l2(out_d.cpu().numpy(), ref) <= TOL
+
+
+def test_repeat_launches_are_bitwise_identical():
+    from trajsparsenufft_amd import device
+    d, _, header = _direction("G_T")
+    _, hi, res = random_panels(header["n_cols"], 8, seed=9)
+    h, r = dev(hi), dev(res)
+    outs = []
+    for _ in range(3):
+        out = torch.zeros((8, header["output_size"]), dtype=torch.complex64, device="cuda")
+        device.interp(d, h, r, out, coils=8, conjugate=True)
+        outs.append(out.cpu().numpy().view(np.uint32))
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+
+
+@pytest.mark.parametrize("view", ["G", "G_T"])
+@pytest.mark.parametrize("coils", [8, 4, 16])
+def test_fp32_csr_matches_oracle(view, coils):
+    from oracle.solver import apply_csr, apply_csr_adjoint
+    from trajsparsenufft_amd import device
+    _, packs = small_case()
+    cols, re, im = packs.kb.arrays()
+    d = device.Direction.csr(packs.kb, view)
+    rng = np.random.default_rng(coils)
+    x = (rng.standard_normal((d.cols, coils)) + 1j * rng.standard_normal((d.cols, coils))).astype(np.complex64)
+    out = torch.zeros((coils, d.rows), dtype=torch.complex64, device="cuda")
+    device.interp_f32(d, dev(x), out, coils=coils, conjugate=view == "G_T", scale=2.0)
+    if view == "G":
+        ref = 2.0 * apply_csr(cols, re, im, x.astype(np.complex128)).T
+    else:
+        ref = 2.0 * apply_csr_adjoint(cols, re, im, x.astype(np.complex128), d.rows).T
+    assert rel_l2(out.cpu().numpy(), ref) <= 1e-6
+
+
+def test_split_panel_is_bitwise_numpy_rounding():
+    from trajsparsenufft_amd import device
+    rng = np.random.default_rng(3)
+    coils, rows = 12, 1000
+    v = (rng.standard_normal((coils, rows)) + 1j * rng.standard_normal((coils, rows))).astype(np.complex64)
+    v[0, :10] = [0, 1e-9, -1e-9, 65504, 1e-5, 3.14159, -0.0, 6e-8, 1e4, 2.5e-3]
+    hi, res = device.split_panel(dev(v), coils, scale=0.25)
+    hi, res = hi.cpu().numpy(), res.cpu().numpy()
+    s = v * np.float32(0.25)
+    for b in range(2):
+        for j in range(8):
+            c = 8 * b + j
+            for part, ch in ((s.real, j), (s.imag, j + 8)):
+                if c < coils:
+                    x = part[c].astype(np.float32)
+                    h = x.astype(np.float16)
+                    r = (x - h.astype(np.float32)).astype(np.float16)
+                else:
+                    h = r = np.zeros(rows, np.float16)
+                assert np.array_equal(hi[b, :, ch].view(np.uint16), h.view(np.uint16))
+                assert np.array_equal(res[b, :, ch].view(np.uint16), r.view(np.uint16))
+
+
+def test_adjointness_of_packed_pair():
+    """<G x, y> = <x, G^H y> through the two independent packs (FP16 tiles)."""
+
+    from trajsparsenufft_amd import device
+    from trajsparsenufft_amd.device import Direction
+    _, packs = small_case()
+    fwd, adj = Direction.upload(packs.forward), Direction.upload(packs.adjoint)
+    rng = np.random.default_rng(11)
+    _, xh, xr = random_panels(fwd.cols, 8, seed=1)
+    _, yh, yr = random_panels(fwd.rows, 8, seed=2)
+    gx = torch.zeros((8, fwd.rows), dtype=torch.complex64, device="cuda")
+    ghy = torch.zeros((8, adj.rows), dtype=torch.complex64, device="cuda")
+    device.interp(fwd, dev(xh), dev(xr), gx, coils=8)
+    device.interp(adj, dev(yh), dev(yr), ghy, coils=8, conjugate=True)
+    x = (xh.astype(np.float64) + xr.astype(np.float64))[0]
+    y = (yh.astype(np.float64) + yr.astype(np.float64))[0]
+    xc = x[:, :8] + 1j * x[:, 8:]
+    yc = y[:, :8] + 1j * y[:, 8:]
+    left = np.vdot(gx.cpu().numpy().T.astype(np.complex128), yc)
+    right = np.vdot(xc, ghy.cpu().numpy().T.astype(np.complex128))
+    assert abs(left - right) / max(abs(left), abs(right)) <= 1e-5
+    _ = rng

@@ -1,0 +1,301 @@
+// In-house FFT endpoints for gfx950.
+//
+// One wave64 computes one length-K (K = 64*R, R in {1, 2, 4, 8, 12}) DFT: lane l holds
+// elements l + 64 j; a radix-R pass in registers (zero-padded inputs skipped),
+// then a 64-point DFT as two radix-8 passes through the wave's LDS slice.  A
+// workgroup of eight waves transforms eight coils of one image column or grid
+// row, so every global read and write is a contiguous block of rows with all
+// coils interleaved.  Fused: SENSE expand x*S_c, apodization, zero-pad (forward
+// first axis), FP16 hi/residual split into the reference's N16 panel layout
+// (forward second axis), crop (inverse, pruned last radix-8 stage), SENSE
+// combine sum_c conj(S_c) img_c, operator scaling, Tikhonov term and the CG dot
+// partial <p, Ap> (inverse second axis).
+//
+// Forward model (reference nufft_custom_fft_forward_bench.cu:1600-1844,
+// 538-747; semantics of torchkbnufft fft_and_scale): grid = DFT2(pad(S_c x s)),
+// sign -1, unnormalised, no shift, grid row index k0*K1 + k1.  Inverse
+// (reference :1463-1552): image = crop(IDFT2(grid)) unnormalised, scaled by the
+// caller.
+#include "fft_common.h"
+
+namespace tsn {
+
+// ---------------------------------------------------------------- kernels
+
+// Forward, first axis: image column n1 (all n0 < N0) of x*S_c*apod for eight
+// coils -> T[k0][n1][c].  Image-space arrays are column-major ([n1][n0]).
+template <int R>
+__global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs a) {
+  extern __shared__ float2 lds[];
+  constexpr int K = 64 * R;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t n1 = blockIdx.x, chunk = blockIdx.y;
+  const uint32_t c = chunk * 8 + w;
+  const bool active = c < a.coils;
+  float2 x[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) x[j] = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const uint32_t n0 = lane + 64 * j;
+    if (active && n0 < a.n0) {
+      const size_t pix = static_cast<size_t>(n1) * a.n0 + n0;
+      const float2 v = cmul(a.image_t[pix], a.smaps_t[(static_cast<size_t>(n1) * a.coils + c) * a.n0 + n0]);
+      const float s = a.apod_t[pix];
+      x[j] = make_float2(v.x * s, v.y * s);
+    }
+  }
+  constexpr int TPL = (R * 8 + 63) / 64;
+  float2 out[TPL][8];
+  wave_fft<R, false, true, false>(x, lds + w * wave_lds_elems<R>(), a.tw0, lane, out);
+  __syncthreads();
+  emit_exchange<R, false>(out, lds, lane, w);
+  __syncthreads();
+  const uint32_t cw = min(8u, a.coils - chunk * 8);
+  for (uint32_t k = threadIdx.x; k < static_cast<uint32_t>(K); k += blockDim.x) {
+    float2* dst = a.t_fwd + (static_cast<size_t>(k) * a.n1 + n1) * a.coils + chunk * 8;
+    const float2* src = lds + k * 8;
+    if (cw == 8) {
+      float4* d4 = reinterpret_cast<float4*>(dst);
+      const float4* s4 = reinterpret_cast<const float4*>(src);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) d4[i] = s4[i];
+    } else {
+      for (uint32_t i = 0; i < cw; ++i) dst[i] = src[i];
+    }
+  }
+}
+
+// Forward, second axis: grid row k0 -> panel rows k0*K1 + k1.  OUT 0: FP16
+// hi/residual N16 panel blocks; OUT 1: float2 [row][coils].
+template <int R, int OUT>
+__global__ __launch_bounds__(512) void fwd_axis1_kernel(FftArgs a) {
+  extern __shared__ float2 lds[];
+  constexpr int K = 64 * R;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t k0 = blockIdx.x, chunk = blockIdx.y;
+  const uint32_t cw = min(8u, a.coils - chunk * 8);
+  for (uint32_t e = threadIdx.x; e < a.n1 * 8; e += blockDim.x) {
+    const uint32_t n1 = e >> 3, c = e & 7;
+    lds[e] = c < cw ? a.t_fwd[(static_cast<size_t>(k0) * a.n1 + n1) * a.coils + chunk * 8 + c]
+                    : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2 x[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const uint32_t n1 = lane + 64 * j;
+    x[j] = n1 < a.n1 ? lds[n1 * 8 + w] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  constexpr int TPL = (R * 8 + 63) / 64;
+  float2 out[TPL][8];
+  wave_fft<R, false, true, false>(x, lds + w * wave_lds_elems<R>(), a.tw1, lane, out);
+  __syncthreads();
+  emit_exchange<R, false>(out, lds, lane, w);
+  __syncthreads();
+  const uint64_t grid_rows = static_cast<uint64_t>(a.k0) * K;
+  for (uint32_t k1 = threadIdx.x; k1 < static_cast<uint32_t>(K); k1 += blockDim.x) {
+    const uint64_t r = static_cast<uint64_t>(k0) * K + k1;
+    const float2* src = lds + k1 * 8;
+    if constexpr (OUT == 0) {
+      half8 hre, him, rre, rim;  // channels 0..7 = Re coils, 8..15 = Im coils
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const float2 v = src[c];
+        _Float16 h, r2;
+        split16(v.x, h, r2);
+        hre[c] = h;
+        rre[c] = r2;
+        split16(v.y, h, r2);
+        him[c] = h;
+        rim[c] = r2;
+      }
+      uint4* dh = reinterpret_cast<uint4*>(a.panel_hi + (chunk * grid_rows + r) * 16);
+      uint4* dr = reinterpret_cast<uint4*>(a.panel_res + (chunk * grid_rows + r) * 16);
+      dh[0] = __builtin_bit_cast(uint4, hre);
+      dh[1] = __builtin_bit_cast(uint4, him);
+      dr[0] = __builtin_bit_cast(uint4, rre);
+      dr[1] = __builtin_bit_cast(uint4, rim);
+    } else {
+      float2* dst = a.panel32 + r * a.coils + chunk * 8;
+      for (uint32_t i = 0; i < cw; ++i) dst[i] = src[i];
+    }
+  }
+}
+
+// Inverse, second grid axis: grid row k0 ([k0*K1 + k1][coils]) -> IDFT along
+// k1, keep n1 < N1 -> T2[n1][k0][c].
+template <int R>
+__global__ __launch_bounds__(512) void inv_axis1_kernel(FftArgs a) {
+  extern __shared__ float2 lds[];
+  constexpr int K = 64 * R;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t k0 = blockIdx.x, chunk = blockIdx.y;
+  const uint32_t cw = min(8u, a.coils - chunk * 8);
+  for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(K) * 8; e += blockDim.x) {
+    const uint32_t k1 = e >> 3, c = e & 7;
+    lds[e] = c < cw ? a.grid[(static_cast<size_t>(k0) * K + k1) * a.coils + chunk * 8 + c]
+                    : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2 x[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) x[j] = lds[(lane + 64 * j) * 8 + w];
+  __syncthreads();
+  constexpr int TPL = (R * 8 + 63) / 64;
+  float2 out[TPL][8];
+  wave_fft<R, true, false, true>(x, lds + w * wave_lds_elems<R>(), a.tw1, lane, out);
+  __syncthreads();
+  emit_exchange<R, true>(out, lds, lane, w);
+  __syncthreads();
+  for (uint32_t n1 = threadIdx.x; n1 < a.n1; n1 += blockDim.x) {
+    float2* dst = a.t_inv + (static_cast<size_t>(n1) * a.k0 + k0) * a.coils + chunk * 8;
+    const float2* src = lds + n1 * 8;
+    if (cw == 8) {
+      float4* d4 = reinterpret_cast<float4*>(dst);
+      const float4* s4 = reinterpret_cast<const float4*>(src);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) d4[i] = s4[i];
+    } else {
+      for (uint32_t i = 0; i < cw; ++i) dst[i] = src[i];
+    }
+  }
+}
+
+// Inverse, first grid axis for image column n1: IDFT along k0, keep n0 < N0,
+// SENSE combine over all coils, * apod * out_scale, + lambda * reg, and the
+// block's partial Re<reg, out> (CG denominator) in float64.
+template <int R>
+__global__ __launch_bounds__(512) void inv_axis0_kernel(FftArgs a) {
+  extern __shared__ float2 lds[];
+  __shared__ double red[512];
+  constexpr int K = 64 * R;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t n1 = blockIdx.x;
+  float2 acc = make_float2(0.f, 0.f);
+  const uint32_t chunks = (a.coils + 7) / 8;
+  for (uint32_t chunk = 0; chunk < chunks; ++chunk) {
+    const uint32_t cw = min(8u, a.coils - chunk * 8);
+    for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(K) * 8; e += blockDim.x) {
+      const uint32_t k0 = e >> 3, c = e & 7;
+      lds[e] = c < cw ? a.t_inv[(static_cast<size_t>(n1) * K + k0) * a.coils + chunk * 8 + c]
+                      : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    float2 x[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) x[j] = lds[(lane + 64 * j) * 8 + w];
+    __syncthreads();
+    constexpr int TPL = (R * 8 + 63) / 64;
+    float2 out[TPL][8];
+    wave_fft<R, true, false, true>(x, lds + w * wave_lds_elems<R>(), a.tw0, lane, out);
+    // multiply by conj(S_c[n0]) before the coil sum
+    const uint32_t c = chunk * 8 + w;
+    const float2* sm = a.smaps_t + (static_cast<size_t>(n1) * a.coils + min(c, a.coils - 1)) * a.n0;
+#pragma unroll
+    for (int s = 0; s < TPL; ++s) {
+      const int task = lane + 64 * s;
+      if (task >= R * 8) continue;
+      const int k1 = task >> 3, k2a = task & 7;
+#pragma unroll
+      for (int k2b = 0; k2b < 4; ++k2b) {
+        const int n0 = k1 + R * (k2a + 8 * k2b);
+        out[s][k2b] = (w < static_cast<int>(cw)) ? cmul_conj_b(out[s][k2b], sm[n0]) : make_float2(0.f, 0.f);
+      }
+    }
+    __syncthreads();
+    emit_exchange<R, true>(out, lds, lane, w);
+    __syncthreads();
+    if (threadIdx.x < a.n0) {
+      const float2* src = lds + threadIdx.x * 8;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        acc.x += src[i].x;
+        acc.y += src[i].y;
+      }
+    }
+    __syncthreads();
+  }
+  double part = 0.0;
+  if (threadIdx.x < a.n0) {
+    const size_t pix = static_cast<size_t>(n1) * a.n0 + threadIdx.x;
+    const float s = a.apod_t[pix] * a.out_scale;
+    float2 v = make_float2(acc.x * s, acc.y * s);
+    if (a.reg_in) {
+      const float2 p = a.reg_in[pix];
+      v.x += a.lambda * p.x;
+      v.y += a.lambda * p.y;
+      part = static_cast<double>(p.x) * v.x + static_cast<double>(p.y) * v.y;
+    }
+    a.image_out[pix] = v;
+  }
+  if (a.dot_partial) {
+    red[threadIdx.x] = part;
+    __syncthreads();
+    for (int off = 256; off > 0; off >>= 1) {
+      if (static_cast<int>(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) a.dot_partial[n1] = red[0];
+  }
+}
+
+// ------------------------------------------------------------- launchers
+
+template <int R>
+static size_t lds_bytes_for() {
+  return static_cast<size_t>(8) * wave_lds_elems<R>() * sizeof(float2);
+}
+
+static int radix_of(uint32_t k) {
+  switch (k) {
+    case 64: return 1;
+    case 128: return 2;
+    case 256: return 4;
+    case 512: return 8;
+    case 768: return 12;
+    default: return 0;
+  }
+}
+
+#define TSN_DISPATCH_R(Rv, EXPR) \
+  switch (Rv) {                  \
+    case 1: { constexpr int R = 1; EXPR; break; }   \
+    case 2: { constexpr int R = 2; EXPR; break; }   \
+    case 4: { constexpr int R = 4; EXPR; break; }   \
+    case 8: { constexpr int R = 8; EXPR; break; }   \
+    case 12: { constexpr int R = 12; EXPR; break; } \
+    default: return fail(TSN_ERR_ARGUMENT, "unsupported FFT length"); \
+  }
+
+int fft_forward(const FftArgs& a, int out_kind, hipStream_t s) {
+  const int r0 = radix_of(a.k0), r1 = radix_of(a.k1);
+  const uint32_t chunks = (a.coils + 7) / 8;
+  TSN_DISPATCH_R(r0, hipLaunchKernelGGL(fwd_axis0_kernel<R>, dim3(a.n1, chunks), dim3(512),
+                                        lds_bytes_for<R>(), s, a));
+  if (out_kind == 0) {
+    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 0>), dim3(a.k0, chunks), dim3(512),
+                                          lds_bytes_for<R>(), s, a));
+  } else {
+    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 1>), dim3(a.k0, chunks), dim3(512),
+                                          lds_bytes_for<R>(), s, a));
+  }
+  TSN_HIP(hipGetLastError());
+  return TSN_OK;
+}
+
+int fft_inverse(const FftArgs& a, hipStream_t s) {
+  const int r0 = radix_of(a.k0), r1 = radix_of(a.k1);
+  const uint32_t chunks = (a.coils + 7) / 8;
+  TSN_DISPATCH_R(r1, hipLaunchKernelGGL(inv_axis1_kernel<R>, dim3(a.k0, chunks), dim3(512),
+                                        lds_bytes_for<R>(), s, a));
+  TSN_DISPATCH_R(r0, hipLaunchKernelGGL(inv_axis0_kernel<R>, dim3(a.n1), dim3(512),
+                                        lds_bytes_for<R>(), s, a));
+  TSN_HIP(hipGetLastError());
+  return TSN_OK;
+}
+
+bool fft_length_supported(uint32_t k) { return radix_of(k) != 0; }
+
+}  // namespace tsn

@@ -1,0 +1,224 @@
+// Device building blocks of the wave64 FFT (shared by the plan kernels and
+// the reference-ABI endpoints).
+#pragma once
+
+#include "tsn_internal.h"
+
+namespace tsn {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+
+struct FftArgs {
+  uint32_t n0, n1;         // image shape
+  uint32_t k0, k1;         // grid shape
+  uint32_t coils;
+  const float2* tw0;       // W_K0^k, k < K0
+  const float2* tw1;       // W_K1^k
+  const float2* image_t;   // [n1][n0]
+  const float2* smaps_t;   // [n1][coil][n0]
+  const float* apod_t;     // [n1][n0]
+  float2* t_fwd;           // [K0][n1][coil]
+  _Float16* panel_hi;      // [block][K0*K1][16]
+  _Float16* panel_res;
+  float2* panel32;         // [K0*K1][coil]
+  const float2* grid;      // [K0*K1][coil]
+  float2* t_inv;           // [n1][K0][coil]
+  float2* image_out;       // [n1][n0]
+  const float2* reg_in;    // [n1][n0] or null
+  float lambda;
+  float out_scale;
+  double* dot_partial;     // [n1] or null
+};
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmul_conj_b(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+// twiddle multiply: forward W, inverse conj(W)
+template <bool INV>
+__device__ __forceinline__ float2 twiddle(float2 a, float2 w) {
+  return INV ? cmul_conj_b(a, w) : cmul(a, w);
+}
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 mul_mi(float2 a) {
+  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+__device__ __forceinline__ void split16(float v, _Float16& hi, _Float16& res) {
+  hi = static_cast<_Float16>(v);
+  res = static_cast<_Float16>(v - static_cast<float>(hi));
+}
+
+// 2-point DFT; NZ = number of leading nonzero inputs (1 or 2).
+template <bool INV, int NZ>
+__device__ __forceinline__ void dft2(const float2* a, float2* X) {
+  if constexpr (NZ == 2) {
+    X[0] = cadd(a[0], a[1]);
+    X[1] = csub(a[0], a[1]);
+  } else {
+    X[0] = a[0];
+    X[1] = a[0];
+  }
+}
+
+// 4-point DFT; NZ = number of leading nonzero inputs (2 or 4).
+template <bool INV, int NZ>
+__device__ __forceinline__ void dft4(const float2* a, float2* X) {
+  float2 t0, t1, t2, t3;
+  if constexpr (NZ == 4) {
+    t0 = cadd(a[0], a[2]);
+    t1 = csub(a[0], a[2]);
+    t2 = cadd(a[1], a[3]);
+    t3 = mul_mi<INV>(csub(a[1], a[3]));
+  } else {
+    t0 = a[0];
+    t1 = a[0];
+    t2 = a[1];
+    t3 = mul_mi<INV>(a[1]);
+  }
+  X[0] = cadd(t0, t2);
+  X[2] = csub(t0, t2);
+  X[1] = cadd(t1, t3);
+  X[3] = csub(t1, t3);
+}
+
+// 8-point DFT; NZ = number of leading nonzero inputs (4 or 8).
+template <bool INV, int NZ>
+__device__ __forceinline__ void dft8(const float2* a, float2* X) {
+  constexpr float h = 0.70710678118654752440f;
+  float2 e_in[4] = {a[0], a[2], a[4], a[6]};
+  float2 o_in[4] = {a[1], a[3], a[5], a[7]};
+  float2 E[4], O[4];
+  dft4<INV, NZ / 2>(e_in, E);
+  dft4<INV, NZ / 2>(o_in, O);
+  // W8^1 O1, W8^2 O2, W8^3 O3 (forward W8 = e^{-i pi/4}; inverse conjugate)
+  const float2 o1 = INV ? make_float2(h * (O[1].x - O[1].y), h * (O[1].x + O[1].y))
+                        : make_float2(h * (O[1].x + O[1].y), h * (O[1].y - O[1].x));
+  const float2 o2 = mul_mi<INV>(O[2]);
+  const float2 o3 = INV ? make_float2(-h * (O[3].x + O[3].y), h * (O[3].x - O[3].y))
+                        : make_float2(h * (O[3].y - O[3].x), -h * (O[3].x + O[3].y));
+  X[0] = cadd(E[0], O[0]);
+  X[4] = csub(E[0], O[0]);
+  X[1] = cadd(E[1], o1);
+  X[5] = csub(E[1], o1);
+  X[2] = cadd(E[2], o2);
+  X[6] = csub(E[2], o2);
+  X[3] = cadd(E[3], o3);
+  X[7] = csub(E[3], o3);
+}
+
+// Per-wave LDS slice (float2 elements): pass-A layout R rows of 65, pass-B/C
+// layout (R*8 tasks) x 9; both padded against bank conflicts.
+template <int R>
+__host__ __device__ constexpr int wave_lds_elems() {
+  return (R * 72 > R * 65 ? R * 72 : R * 65);
+}
+
+// Length K = 64*R DFT of one wave, R in {1, 2, 4, 8, 12}.  x[j] = element
+// lane + 64 j (elements >= K/2 are zero when HALF_IN; the caller loads zeros
+// there, and for even R the radix-R pass skips them).  Output: out[s][k2b] = X[k1 + R*(k2a + 8*k2b)]
+// for task = lane + 64 s < 8R, k1 = task >> 3, k2a = task & 7 (only k2b < 4,
+// i.e. k < K/2, when HALF_OUT).  tw = W_K^k table.  Uses __syncthreads: all
+// waves of the block must call it together.
+template <int R, bool INV, bool HALF_IN, bool HALF_OUT>
+__device__ __forceinline__ void wave_fft(float2 (&x)[R], float2* buf, const float2* __restrict__ tw,
+                                         int lane, float2 (&out)[(R * 8 + 63) / 64][8]) {
+  constexpr int K = 64 * R;
+  constexpr int TPL = (R * 8 + 63) / 64;
+  static_assert(R == 1 || R == 2 || R == 4 || R == 8 || R == 12, "unsupported radix");
+  // pass A: radix-R over j, twiddle W_K^{lane*k1}
+  float2 y[R];
+  if constexpr (R == 1) {
+    y[0] = x[0];
+  } else if constexpr (R == 2) {
+    dft2<INV, HALF_IN ? 1 : 2>(x, y);
+  } else if constexpr (R == 8) {
+    dft8<INV, HALF_IN ? 4 : 8>(x, y);
+  } else if constexpr (R == 4) {
+    dft4<INV, HALF_IN ? 2 : 4>(x, y);
+  } else {
+    constexpr int JN = HALF_IN ? R / 2 : R;
+#pragma unroll
+    for (int k1 = 0; k1 < R; ++k1) {
+      float2 s = x[0];
+#pragma unroll
+      for (int j = 1; j < JN; ++j) s = cadd(s, twiddle<INV>(x[j], tw[((j * k1) % R) * 64]));
+      y[k1] = s;
+    }
+  }
+#pragma unroll
+  for (int k1 = 1; k1 < R; ++k1) y[k1] = twiddle<INV>(y[k1], tw[lane * k1]);
+#pragma unroll
+  for (int k1 = 0; k1 < R; ++k1) buf[k1 * 65 + lane] = y[k1];
+  __syncthreads();
+  // pass B: task (k1, b0): radix-8 over b1, twiddle W_64^{b0*k2a}
+  float2 u[TPL][8];
+#pragma unroll
+  for (int s = 0; s < TPL; ++s) {
+    const int task = lane + 64 * s;
+    if (task < R * 8) {
+      const int k1 = task >> 3, b0 = task & 7;
+#pragma unroll
+      for (int b1 = 0; b1 < 8; ++b1) u[s][b1] = buf[k1 * 65 + b0 + 8 * b1];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < TPL; ++s) {
+    const int task = lane + 64 * s;
+    if (task < R * 8) {
+      const int k1 = task >> 3, b0 = task & 7;
+      float2 v[8];
+      dft8<INV, 8>(u[s], v);
+#pragma unroll
+      for (int k2a = 1; k2a < 8; ++k2a) v[k2a] = twiddle<INV>(v[k2a], tw[R * b0 * k2a]);
+#pragma unroll
+      for (int k2a = 0; k2a < 8; ++k2a) buf[(k1 * 8 + k2a) * 9 + b0] = v[k2a];
+    }
+  }
+  __syncthreads();
+  // pass C: task (k1, k2a): radix-8 over b0
+#pragma unroll
+  for (int s = 0; s < TPL; ++s) {
+    const int task = lane + 64 * s;
+    if (task < R * 8) {
+      float2 v[8];
+#pragma unroll
+      for (int b0 = 0; b0 < 8; ++b0) v[b0] = buf[task * 9 + b0];
+      dft8<INV, 8>(v, out[s]);
+    }
+  }
+  (void)K;
+  (void)HALF_OUT;
+}
+
+// Write the wave's outputs to the block exchange area lds[k*8 + wave]
+// (k < K, or k < K/2 when HALF_OUT).  Call between two __syncthreads.
+template <int R, bool HALF_OUT>
+__device__ __forceinline__ void emit_exchange(const float2 (&out)[(R * 8 + 63) / 64][8], float2* lds,
+                                              int lane, int w) {
+  constexpr int TPL = (R * 8 + 63) / 64;
+#pragma unroll
+  for (int s = 0; s < TPL; ++s) {
+    const int task = lane + 64 * s;
+    if (task >= R * 8) continue;
+    const int k1 = task >> 3, k2a = task & 7;
+#pragma unroll
+    for (int k2b = 0; k2b < (HALF_OUT ? 4 : 8); ++k2b) {
+      const int k = k1 + R * (k2a + 8 * k2b);
+      lds[k * 8 + w] = out[s][k2b];
+    }
+  }
+}
+
+int fft_forward(const FftArgs& a, int out_kind, hipStream_t s);
+int fft_inverse(const FftArgs& a, hipStream_t s);
+bool fft_length_supported(uint32_t k);
+
+}  // namespace tsn

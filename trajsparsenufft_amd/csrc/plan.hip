@@ -1,0 +1,580 @@
+// Reconstruction plan: operators A, A^H, A^H A and CG-SENSE with float64
+// scalars, eager or captured as one hipGraph.
+//
+// Schedule and conventions follow the reference production binary
+// (experiments/production/src/nufft_fp16x2_cg_bench.cu):
+//   measurement / sqrt(K0*K1)                     :329-334
+//   forward interpolation * 1/sqrt(K0*K1)         :230-236
+//   inverse FFT * apod * sqrt(K0K1)/(K0K1)        :335-352 (prenormalised)
+//   normal = combine * 1/s^2 + lambda/(K0K1)/s^2  :27-54, 401-404
+//   CG update order, curvature flag (alpha = 0)   :56-180, 241-268
+// Differences by design: float64 CG scalars (rho, <p,Ap>, alpha, beta, history);
+// image vectors kept column-major [n1][n0] inside the plan; SENSE combine,
+// Tikhonov term and the <p, Ap> partial fused into the last inverse FFT pass;
+// the forward interpolation writes the adjoint's FP16 hi/res sample panels
+// directly (no float2 sample round trip).
+#include <cmath>
+#include <mutex>
+
+#include "fft_common.h"
+
+namespace tsn {
+int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* res, uint32_t coils,
+                  int out_mode, bool conj, bool accumulate, float scale, float* out_f32,
+                  _Float16* out_hi, _Float16* out_res, hipStream_t stream);
+int launch_interp_csr(const tsn_direction* d, const float2* in, uint32_t coils, int out_mode,
+                      bool conj, bool accumulate, float scale, float2* out, hipStream_t stream);
+int launch_split_panel(const float2* in, uint32_t rows, uint32_t coils, float scale, _Float16* hi,
+                       _Float16* res, hipStream_t stream);
+}  // namespace tsn
+
+using namespace tsn;
+
+namespace {
+
+constexpr int kVecThreads = 256;
+constexpr int kVecPerThread = 4;
+
+struct CgScalars {
+  double rho;
+  double denominator;
+  double alpha;
+  double beta;
+};
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (static_cast<int>(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+// x = 0, r = p = b, partial <r, r>
+__global__ __launch_bounds__(kVecThreads) void cg_init_kernel(const float2* __restrict__ b,
+                                                              float2* __restrict__ x, float2* __restrict__ r,
+                                                              float2* __restrict__ p, uint32_t n,
+                                                              double* __restrict__ partial) {
+  __shared__ double red[kVecThreads];
+  double s = 0.0;
+  const uint32_t base = blockIdx.x * kVecThreads * kVecPerThread + threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < kVecPerThread; ++i) {
+    const uint32_t e = base + i * kVecThreads;
+    if (e < n) {
+      const float2 v = b[e];
+      x[e] = make_float2(0.f, 0.f);
+      r[e] = v;
+      p[e] = v;
+      s += static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y;
+    }
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// x += alpha p ; r -= alpha Ap ; partial <r, r>
+__global__ __launch_bounds__(kVecThreads) void cg_update_xr_kernel(
+    float2* __restrict__ x, float2* __restrict__ r, const float2* __restrict__ p,
+    const float2* __restrict__ ap, const CgScalars* __restrict__ sc, uint32_t n,
+    double* __restrict__ partial) {
+  __shared__ double red[kVecThreads];
+  const double alpha = sc->alpha;
+  double s = 0.0;
+  const uint32_t base = blockIdx.x * kVecThreads * kVecPerThread + threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < kVecPerThread; ++i) {
+    const uint32_t e = base + i * kVecThreads;
+    if (e < n) {
+      const float2 pv = p[e], av = ap[e];
+      float2 xv = x[e], rv = r[e];
+      xv.x = static_cast<float>(xv.x + alpha * pv.x);
+      xv.y = static_cast<float>(xv.y + alpha * pv.y);
+      rv.x = static_cast<float>(rv.x - alpha * av.x);
+      rv.y = static_cast<float>(rv.y - alpha * av.y);
+      x[e] = xv;
+      r[e] = rv;
+      s += static_cast<double>(rv.x) * rv.x + static_cast<double>(rv.y) * rv.y;
+    }
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// p = r + beta p
+__global__ __launch_bounds__(kVecThreads) void cg_update_p_kernel(float2* __restrict__ p,
+                                                                  const float2* __restrict__ r,
+                                                                  const CgScalars* __restrict__ sc,
+                                                                  uint32_t n) {
+  const double beta = sc->beta;
+  const uint32_t base = blockIdx.x * kVecThreads * kVecPerThread + threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < kVecPerThread; ++i) {
+    const uint32_t e = base + i * kVecThreads;
+    if (e < n) {
+      const float2 rv = r[e], pv = p[e];
+      p[e] = make_float2(static_cast<float>(rv.x + beta * pv.x), static_cast<float>(rv.y + beta * pv.y));
+    }
+  }
+}
+
+enum ReduceMode { kReduceInit = 0, kReduceAlpha = 1, kReduceBeta = 2 };
+
+// One block sums the partials in a fixed order, then updates the scalars.
+__global__ __launch_bounds__(kVecThreads) void cg_reduce_kernel(const double* __restrict__ partial,
+                                                                uint32_t count, int mode, int iteration,
+                                                                CgScalars* __restrict__ sc,
+                                                                double* __restrict__ history,
+                                                                int* __restrict__ status) {
+  __shared__ double red[kVecThreads];
+  double s = 0.0;
+  for (uint32_t i = threadIdx.x; i < count; i += kVecThreads) s += partial[i];
+  s = block_sum(s, red);
+  if (threadIdx.x != 0) return;
+  if (mode == kReduceInit) {
+    sc->rho = s;
+    history[0] = sqrt(fmax(s, 0.0));
+    *status = 1;
+  } else if (mode == kReduceAlpha) {
+    sc->denominator = s;
+    if (!isfinite(s) || s <= 0.0) {
+      *status = 0;
+      sc->alpha = 0.0;
+    } else {
+      sc->alpha = sc->rho / s;
+    }
+  } else {
+    sc->beta = s / fmax(sc->rho, 1e-30);
+    sc->rho = s;
+    history[iteration + 1] = sqrt(fmax(s, 0.0));
+  }
+}
+
+// [rows][cols] -> [cols][rows] complex, 32x32 tiles, optional scale.
+__global__ __launch_bounds__(256) void transpose_kernel(const float2* __restrict__ in, uint32_t rows,
+                                                        uint32_t cols, float scale,
+                                                        float2* __restrict__ out) {
+  __shared__ float2 tile[32][33];
+  const uint32_t bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  for (uint32_t i = threadIdx.y; i < 32; i += 8) {
+    const uint32_t r = by + i, c = bx + threadIdx.x;
+    if (r < rows && c < cols) tile[i][threadIdx.x] = in[static_cast<size_t>(r) * cols + c];
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.y; i < 32; i += 8) {
+    const uint32_t c = bx + i, r = by + threadIdx.x;
+    if (r < rows && c < cols) {
+      const float2 v = tile[threadIdx.x][i];
+      out[static_cast<size_t>(c) * rows + r] = make_float2(v.x * scale, v.y * scale);
+    }
+  }
+}
+
+// [coils][n0][n1] -> [n1][coils][n0]
+__global__ __launch_bounds__(256) void smaps_transpose_kernel(const float2* __restrict__ in, uint32_t coils,
+                                                              uint32_t n0, uint32_t n1,
+                                                              float2* __restrict__ out) {
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t total = static_cast<uint64_t>(coils) * n0 * n1;
+  if (e >= total) return;
+  const uint32_t x1 = static_cast<uint32_t>(e % n1);
+  const uint32_t x0 = static_cast<uint32_t>((e / n1) % n0);
+  const uint32_t c = static_cast<uint32_t>(e / (static_cast<uint64_t>(n1) * n0));
+  out[(static_cast<size_t>(x1) * coils + c) * n0 + x0] = in[e];
+}
+
+__global__ __launch_bounds__(256) void apod_transpose_kernel(const float* __restrict__ in, uint32_t n0,
+                                                             uint32_t n1, float* __restrict__ out) {
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<uint64_t>(n0) * n1) return;
+  const uint32_t x1 = static_cast<uint32_t>(e % n1), x0 = static_cast<uint32_t>(e / n1);
+  out[static_cast<size_t>(x1) * n0 + x0] = in[e];
+}
+
+__global__ __launch_bounds__(256) void fill_zero_kernel(float4* __restrict__ p, uint64_t n4) {
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < n4) p[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// [coils][m] -> [m][coils] * scale (FP32 path sample layout)
+__global__ __launch_bounds__(256) void samples_to_rows_kernel(const float2* __restrict__ in, uint32_t coils,
+                                                              uint32_t m, float scale,
+                                                              float2* __restrict__ out) {
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<uint64_t>(coils) * m) return;
+  const uint32_t c = static_cast<uint32_t>(e % coils), row = static_cast<uint32_t>(e / coils);
+  const float2 v = in[static_cast<size_t>(c) * m + row];
+  out[e] = make_float2(v.x * scale, v.y * scale);
+}
+
+__global__ void copy_status_kernel(const int* __restrict__ src, int* __restrict__ dst) {
+  if (threadIdx.x == 0) *dst = *src;
+}
+
+}  // namespace
+
+struct tsn_plan {
+  tsn_plan_config cfg{};
+  int device = 0;
+  const tsn_direction* fwd = nullptr;
+  const tsn_direction* adj = nullptr;
+  uint32_t m = 0, blocks = 0, pixels = 0, vec_blocks = 0;
+  size_t bytes = 0;
+  std::vector<void*> allocations;
+  float2 *tw0 = nullptr, *tw1 = nullptr, *smaps_t = nullptr, *t_fwd = nullptr, *t_inv = nullptr;
+  float* apod_t = nullptr;
+  _Float16 *pan_hi = nullptr, *pan_res = nullptr, *smp_hi = nullptr, *smp_res = nullptr;
+  float2 *pan32 = nullptr, *smp32 = nullptr, *grid = nullptr;
+  float2 *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *b = nullptr, *tmp = nullptr;
+  double* partial = nullptr;
+  CgScalars* scalars = nullptr;
+  int* status = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+template <typename T>
+int plan_alloc(tsn_plan* pl, T** ptr, size_t count) {
+  *ptr = nullptr;
+  if (count == 0) return TSN_OK;
+  TSN_HIP(hipMalloc(reinterpret_cast<void**>(ptr), count * sizeof(T)));
+  pl->allocations.push_back(*ptr);
+  pl->bytes += count * sizeof(T);
+  return TSN_OK;
+}
+
+int zero_async(void* p, size_t bytes, hipStream_t s) {
+  // a fill kernel (not hipMemsetAsync) so the work can sit inside a captured graph
+  if (bytes % 16) return fail(TSN_ERR_ARGUMENT, "zero fill needs a multiple of 16 bytes");
+  const uint64_t n4 = bytes / 16;
+  if (n4 == 0) return TSN_OK;
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(div_up(n4, 256)), dim3(256), 0, s, static_cast<float4*>(p), n4);
+  TSN_HIP(hipGetLastError());
+  return TSN_OK;
+}
+
+FftArgs fft_args(const tsn_plan* pl) {
+  FftArgs a{};
+  a.n0 = pl->cfg.image0;
+  a.n1 = pl->cfg.image1;
+  a.k0 = pl->cfg.grid0;
+  a.k1 = pl->cfg.grid1;
+  a.coils = pl->cfg.coils;
+  a.tw0 = pl->tw0;
+  a.tw1 = pl->tw1;
+  a.smaps_t = pl->smaps_t;
+  a.apod_t = pl->apod_t;
+  a.t_fwd = pl->t_fwd;
+  a.panel_hi = pl->pan_hi;
+  a.panel_res = pl->pan_res;
+  a.panel32 = pl->pan32;
+  a.grid = pl->grid;
+  a.t_inv = pl->t_inv;
+  return a;
+}
+
+bool is_fp32(const tsn_plan* pl) { return pl->cfg.precision == TSN_PREC_FP32; }
+
+int interp_kind(const tsn_plan* pl) {
+  switch (pl->cfg.precision) {
+    case TSN_PREC_DENSE16: return 1;
+    case TSN_PREC_HI16: return 2;
+    default: return 0;
+  }
+}
+
+float grid_norm(const tsn_plan* pl) {
+  return static_cast<float>(1.0 / std::sqrt(static_cast<double>(pl->cfg.grid0) * pl->cfg.grid1));
+}
+
+// p (column-major image) -> gridded panels
+int op_forward_grid(tsn_plan* pl, const float2* image_t, hipStream_t s) {
+  FftArgs a = fft_args(pl);
+  a.image_t = image_t;
+  return fft_forward(a, is_fp32(pl) ? 1 : 0, s);
+}
+
+// panels -> samples: split panels (internal) or float2 output (coil-major)
+int op_forward_samples(tsn_plan* pl, float2* out_coil_major, hipStream_t s) {
+  const float scale = grid_norm(pl);
+  if (is_fp32(pl)) {
+    if (out_coil_major)
+      return launch_interp_csr(pl->fwd, pl->pan32, pl->cfg.coils, 0, false, false, scale, out_coil_major, s);
+    return launch_interp_csr(pl->fwd, pl->pan32, pl->cfg.coils, 1, false, false, scale, pl->smp32, s);
+  }
+  if (out_coil_major)
+    return launch_interp(pl->fwd, interp_kind(pl), pl->pan_hi, pl->pan_res, pl->cfg.coils, 0, false,
+                         false, scale, reinterpret_cast<float*>(out_coil_major), nullptr, nullptr, s);
+  return launch_interp(pl->fwd, interp_kind(pl), pl->pan_hi, pl->pan_res, pl->cfg.coils, 2, false, false,
+                       scale, nullptr, pl->smp_hi, pl->smp_res, s);
+}
+
+// sample panels -> grid (conjugate transpose interpolation)
+int op_adjoint_grid(tsn_plan* pl, hipStream_t s) {
+  if (is_fp32(pl))
+    return launch_interp_csr(pl->adj, pl->smp32, pl->cfg.coils, 1, true, false, 1.0f, pl->grid, s);
+  return launch_interp(pl->adj, interp_kind(pl), pl->smp_hi, pl->smp_res, pl->cfg.coils, 1, true, false,
+                       1.0f, reinterpret_cast<float*>(pl->grid), nullptr, nullptr, s);
+}
+
+// grid -> image (column-major) with combine, scale, regulariser, dot partial
+int op_adjoint_image(tsn_plan* pl, float2* out_t, const float2* reg, float lambda, float out_scale,
+                     double* dot_partial, hipStream_t s) {
+  FftArgs a = fft_args(pl);
+  a.image_out = out_t;
+  a.reg_in = reg;
+  a.lambda = lambda;
+  a.out_scale = out_scale * grid_norm(pl);
+  a.dot_partial = dot_partial;
+  return fft_inverse(a, s);
+}
+
+int op_normal(tsn_plan* pl, const float2* in_t, float2* out_t, float inv_scale_sq, float lambda,
+              double* dot_partial, hipStream_t s) {
+  TSN_CHECK(op_forward_grid(pl, in_t, s));
+  TSN_CHECK(op_forward_samples(pl, nullptr, s));
+  TSN_CHECK(op_adjoint_grid(pl, s));
+  return op_adjoint_image(pl, out_t, in_t, lambda, inv_scale_sq, dot_partial, s);
+}
+
+// measurement (raw, coil-major) -> scaled sample panels of the adjoint input
+int op_load_samples(tsn_plan* pl, const float2* samples, float scale, hipStream_t s) {
+  if (is_fp32(pl)) {
+    const uint64_t n = static_cast<uint64_t>(pl->cfg.coils) * pl->m;
+    hipLaunchKernelGGL(samples_to_rows_kernel, dim3(div_up(n, 256)), dim3(256), 0, s, samples,
+                       pl->cfg.coils, pl->m, scale, pl->smp32);
+    TSN_HIP(hipGetLastError());
+    return TSN_OK;
+  }
+  return launch_split_panel(samples, pl->m, pl->cfg.coils, scale, pl->smp_hi, pl->smp_res, s);
+}
+
+int transpose_image(const float2* in, uint32_t rows, uint32_t cols, float scale, float2* out, hipStream_t s) {
+  hipLaunchKernelGGL(transpose_kernel, dim3(div_up(cols, 32), div_up(rows, 32)), dim3(32, 8), 0, s, in,
+                     rows, cols, scale, out);
+  TSN_HIP(hipGetLastError());
+  return TSN_OK;
+}
+
+int enqueue_cg(tsn_plan* pl, const float2* measurement, uint32_t iterations, float operator_scale,
+               float lambda, float2* x_out, double* history, int* status_out, hipStream_t s) {
+  const double sq = static_cast<double>(operator_scale) * operator_scale;
+  const float inv_scale_sq = static_cast<float>(1.0 / sq);
+  const double kk = static_cast<double>(pl->cfg.grid0) * pl->cfg.grid1;
+  const float scaled_lambda = static_cast<float>(lambda / kk / sq);
+  const uint32_t n = pl->pixels;
+  // b = A^H (y / sqrt(K0K1)) / s^2
+  TSN_CHECK(op_load_samples(pl, measurement, grid_norm(pl), s));
+  TSN_CHECK(op_adjoint_grid(pl, s));
+  TSN_CHECK(op_adjoint_image(pl, pl->b, nullptr, 0.0f, inv_scale_sq, nullptr, s));
+  hipLaunchKernelGGL(cg_init_kernel, dim3(pl->vec_blocks), dim3(kVecThreads), 0, s, pl->b, pl->x, pl->r,
+                     pl->p, n, pl->partial);
+  hipLaunchKernelGGL(cg_reduce_kernel, dim3(1), dim3(kVecThreads), 0, s, pl->partial, pl->vec_blocks,
+                     static_cast<int>(kReduceInit), 0, pl->scalars, history, pl->status);
+  for (uint32_t it = 0; it < iterations; ++it) {
+    TSN_CHECK(op_normal(pl, pl->p, pl->ap, inv_scale_sq, scaled_lambda, pl->partial, s));
+    hipLaunchKernelGGL(cg_reduce_kernel, dim3(1), dim3(kVecThreads), 0, s, pl->partial, pl->cfg.image1,
+                       static_cast<int>(kReduceAlpha), static_cast<int>(it), pl->scalars, history, pl->status);
+    hipLaunchKernelGGL(cg_update_xr_kernel, dim3(pl->vec_blocks), dim3(kVecThreads), 0, s, pl->x, pl->r,
+                       pl->p, pl->ap, pl->scalars, n, pl->partial);
+    hipLaunchKernelGGL(cg_reduce_kernel, dim3(1), dim3(kVecThreads), 0, s, pl->partial, pl->vec_blocks,
+                       static_cast<int>(kReduceBeta), static_cast<int>(it), pl->scalars, history, pl->status);
+    hipLaunchKernelGGL(cg_update_p_kernel, dim3(pl->vec_blocks), dim3(kVecThreads), 0, s, pl->p, pl->r,
+                       pl->scalars, n);
+  }
+  TSN_CHECK(transpose_image(pl->x, pl->cfg.image1, pl->cfg.image0, 1.0f, x_out, s));
+  hipLaunchKernelGGL(copy_status_kernel, dim3(1), dim3(64), 0, s, pl->status, status_out);
+  TSN_HIP(hipGetLastError());
+  return TSN_OK;
+}
+
+int check_cg_args(tsn_plan* pl, const void* measurement, uint32_t iterations, float operator_scale,
+                  float lambda, const void* x_out, const void* residuals, const void* status) {
+  if (!pl || !measurement || !x_out || !residuals || !status) return fail(TSN_ERR_ARGUMENT, "null argument");
+  if (iterations == 0) return fail(TSN_ERR_ARGUMENT, "cg iterations must be positive");
+  if (!(operator_scale > 0.0f) || !std::isfinite(operator_scale))
+    return fail(TSN_ERR_ARGUMENT, "operator scale must be positive and finite");
+  if (!(lambda >= 0.0f) || !std::isfinite(lambda)) return fail(TSN_ERR_ARGUMENT, "lambda must be >= 0");
+  return TSN_OK;
+}
+
+std::vector<float2> twiddles(uint32_t k) {
+  std::vector<float2> t(k);
+  for (uint32_t i = 0; i < k; ++i) {
+    const double ang = -2.0 * M_PI * static_cast<double>(i) / static_cast<double>(k);
+    t[i] = make_float2(static_cast<float>(std::cos(ang)), static_cast<float>(std::sin(ang)));
+  }
+  return t;
+}
+
+void destroy(tsn_plan* pl) {
+  if (!pl) return;
+  if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+  if (pl->graph) (void)hipGraphDestroy(pl->graph);
+  for (void* p : pl->allocations) (void)hipFree(p);
+  delete pl;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tsn_plan_create(const tsn_plan_config* cfg, const tsn_direction* fwd, const tsn_direction* adj,
+                    const void* smaps, const void* apodization, tsn_plan** out) {
+  if (!cfg || !fwd || !adj || !smaps || !apodization || !out) return fail(TSN_ERR_ARGUMENT, "null argument");
+  *out = nullptr;
+  const tsn_plan_config c = *cfg;
+  if (c.coils == 0 || c.coils > static_cast<uint32_t>(kMaxCoils))
+    return fail(TSN_ERR_ARGUMENT, "coils must be in 1..16");
+  if (c.grid0 != 2 * c.image0 || c.grid1 != 2 * c.image1 || !fft_length_supported(c.grid0) ||
+      !fft_length_supported(c.grid1))
+    return fail(TSN_ERR_ARGUMENT, "grid must be 2x image with grid sides in {64, 128, 256, 512, 768}");
+  if (c.precision < TSN_PREC_SPLIT16 || c.precision > TSN_PREC_HI16)
+    return fail(TSN_ERR_ARGUMENT, "precision %d is not available in this build", c.precision);
+  const bool fp32 = c.precision == TSN_PREC_FP32;
+  if (fwd->csr != fp32 || adj->csr != fp32)
+    return fail(TSN_ERR_ARGUMENT, "FP32 plans need CSR directions, FP16 plans need packed directions");
+  const uint32_t grid = c.grid0 * c.grid1;
+  if (fwd->h.n_cols != grid || adj->h.output_size != grid || fwd->h.output_size != adj->h.n_cols)
+    return fail(TSN_ERR_INCOMPATIBLE, "unexpected operator shape");
+  if (c.precision == TSN_PREC_DENSE16 && (!fwd->tile_dense || !adj->tile_dense))
+    return fail(TSN_ERR_ARGUMENT, "dense control needs dense tiles");
+  auto* pl = new tsn_plan();
+  pl->cfg = c;
+  (void)hipGetDevice(&pl->device);
+  pl->fwd = fwd;
+  pl->adj = adj;
+  pl->m = fwd->h.output_size;
+  pl->blocks = div_up(c.coils, kCoilsPerBlock);
+  pl->pixels = c.image0 * c.image1;
+  pl->vec_blocks = div_up(pl->pixels, kVecThreads * kVecPerThread);
+  const size_t px = pl->pixels;
+  int rc = TSN_OK;
+  auto A = [&](auto** p, size_t count) {
+    if (rc == TSN_OK) rc = plan_alloc(pl, p, count);
+  };
+  A(&pl->tw0, c.grid0);
+  A(&pl->tw1, c.grid1);
+  A(&pl->smaps_t, px * c.coils);
+  A(&pl->apod_t, px);
+  A(&pl->t_fwd, static_cast<size_t>(c.grid0) * c.image1 * c.coils);
+  A(&pl->t_inv, static_cast<size_t>(c.image1) * c.grid0 * c.coils);
+  A(&pl->grid, static_cast<size_t>(grid) * c.coils);
+  if (fp32) {
+    A(&pl->pan32, static_cast<size_t>(grid) * c.coils);
+    A(&pl->smp32, static_cast<size_t>(pl->m) * c.coils);
+  } else {
+    A(&pl->pan_hi, static_cast<size_t>(pl->blocks) * grid * kN);
+    A(&pl->pan_res, static_cast<size_t>(pl->blocks) * grid * kN);
+    A(&pl->smp_hi, static_cast<size_t>(pl->blocks) * pl->m * kN);
+    A(&pl->smp_res, static_cast<size_t>(pl->blocks) * pl->m * kN);
+  }
+  for (float2** v : {&pl->x, &pl->r, &pl->p, &pl->ap, &pl->b, &pl->tmp}) A(v, px);
+  A(&pl->partial, std::max<size_t>(pl->vec_blocks, c.image1));
+  A(&pl->scalars, 1);
+  A(&pl->status, 1);
+  if (rc == TSN_OK) {
+    const auto t0 = twiddles(c.grid0), t1 = twiddles(c.grid1);
+    const hipError_t e0 = hipMemcpy(pl->tw0, t0.data(), t0.size() * sizeof(float2), hipMemcpyHostToDevice);
+    const hipError_t e1 = hipMemcpy(pl->tw1, t1.data(), t1.size() * sizeof(float2), hipMemcpyHostToDevice);
+    if (e0 != hipSuccess || e1 != hipSuccess) rc = fail(TSN_ERR_RUNTIME, "twiddle upload failed");
+  }
+  if (rc == TSN_OK) {
+    const uint64_t ns = static_cast<uint64_t>(c.coils) * px;
+    hipLaunchKernelGGL(smaps_transpose_kernel, dim3(div_up(ns, 256)), dim3(256), 0, 0,
+                       static_cast<const float2*>(smaps), c.coils, c.image0, c.image1, pl->smaps_t);
+    hipLaunchKernelGGL(apod_transpose_kernel, dim3(div_up(px, 256)), dim3(256), 0, 0,
+                       static_cast<const float*>(apodization), c.image0, c.image1, pl->apod_t);
+    // the adjoint grid's inactive rows are never written: zero it once here
+    rc = zero_async(pl->grid, static_cast<size_t>(grid) * c.coils * sizeof(float2), 0);
+    if (rc == TSN_OK) {
+      const hipError_t e = hipDeviceSynchronize();
+      if (e != hipSuccess) rc = fail(TSN_ERR_RUNTIME, "plan setup: %s", hipGetErrorString(e));
+    }
+  }
+  if (rc != TSN_OK) {
+    destroy(pl);
+    return rc;
+  }
+  *out = pl;
+  return TSN_OK;
+}
+
+void tsn_plan_free(tsn_plan* pl) { destroy(pl); }
+
+size_t tsn_plan_bytes(const tsn_plan* pl) { return pl ? pl->bytes : 0; }
+
+int tsn_forward(tsn_plan* pl, const void* image, void* samples, void* stream) {
+  if (!pl || !image || !samples) return fail(TSN_ERR_ARGUMENT, "null argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  TSN_CHECK(transpose_image(static_cast<const float2*>(image), pl->cfg.image0, pl->cfg.image1, 1.0f, pl->tmp, s));
+  TSN_CHECK(op_forward_grid(pl, pl->tmp, s));
+  return op_forward_samples(pl, static_cast<float2*>(samples), s);
+}
+
+int tsn_adjoint(tsn_plan* pl, const void* samples, void* image, void* stream) {
+  if (!pl || !image || !samples) return fail(TSN_ERR_ARGUMENT, "null argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  TSN_CHECK(op_load_samples(pl, static_cast<const float2*>(samples), 1.0f, s));
+  TSN_CHECK(op_adjoint_grid(pl, s));
+  TSN_CHECK(op_adjoint_image(pl, pl->tmp, nullptr, 0.0f, 1.0f, nullptr, s));
+  return transpose_image(pl->tmp, pl->cfg.image1, pl->cfg.image0, 1.0f, static_cast<float2*>(image), s);
+}
+
+int tsn_normal(tsn_plan* pl, const void* image, void* out, float inv_scale_sq, float scaled_lambda,
+               void* stream) {
+  if (!pl || !image || !out) return fail(TSN_ERR_ARGUMENT, "null argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  TSN_CHECK(transpose_image(static_cast<const float2*>(image), pl->cfg.image0, pl->cfg.image1, 1.0f, pl->tmp, s));
+  TSN_CHECK(op_normal(pl, pl->tmp, pl->ap, inv_scale_sq, scaled_lambda, nullptr, s));
+  return transpose_image(pl->ap, pl->cfg.image1, pl->cfg.image0, 1.0f, static_cast<float2*>(out), s);
+}
+
+int tsn_cg(tsn_plan* pl, const void* measurement, uint32_t iterations, float operator_scale, float lambda,
+           void* x_out, void* residuals_out, void* status_out, void* stream) {
+  TSN_CHECK(check_cg_args(pl, measurement, iterations, operator_scale, lambda, x_out, residuals_out, status_out));
+  return enqueue_cg(pl, static_cast<const float2*>(measurement), iterations, operator_scale, lambda,
+                    static_cast<float2*>(x_out), static_cast<double*>(residuals_out),
+                    static_cast<int*>(status_out), static_cast<hipStream_t>(stream));
+}
+
+int tsn_cg_graph_capture(tsn_plan* pl, const void* measurement, uint32_t iterations, float operator_scale,
+                         float lambda, void* x_out, void* residuals_out, void* status_out, void* stream) {
+  TSN_CHECK(check_cg_args(pl, measurement, iterations, operator_scale, lambda, x_out, residuals_out, status_out));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!s) return fail(TSN_ERR_ARGUMENT, "graph capture needs a non-null stream");
+  if (pl->exec) {
+    (void)hipGraphExecDestroy(pl->exec);
+    pl->exec = nullptr;
+  }
+  if (pl->graph) {
+    (void)hipGraphDestroy(pl->graph);
+    pl->graph = nullptr;
+  }
+  TSN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeGlobal));
+  const int rc = enqueue_cg(pl, static_cast<const float2*>(measurement), iterations, operator_scale, lambda,
+                            static_cast<float2*>(x_out), static_cast<double*>(residuals_out),
+                            static_cast<int*>(status_out), s);
+  hipGraph_t g = nullptr;
+  const hipError_t end = hipStreamEndCapture(s, &g);
+  if (rc != TSN_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (end != hipSuccess) return fail(TSN_ERR_RUNTIME, "hipStreamEndCapture: %s", hipGetErrorString(end));
+  pl->graph = g;
+  TSN_HIP(hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0));
+  return TSN_OK;
+}
+
+int tsn_cg_graph_launch(tsn_plan* pl, void* stream) {
+  if (!pl || !pl->exec) return fail(TSN_ERR_ARGUMENT, "no captured graph");
+  TSN_HIP(hipGraphLaunch(pl->exec, static_cast<hipStream_t>(stream)));
+  return TSN_OK;
+}
+
+}  // extern "C"

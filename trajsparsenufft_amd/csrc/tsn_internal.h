@@ -1,0 +1,119 @@
+// Internal declarations of the gfx950 NUFFT / CG-SENSE library (libtsn.so).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tsn.h"
+
+namespace tsn {
+
+constexpr uint64_t kMagic = 0x4E55464654323431ull;  // "NUFFT241"
+constexpr uint32_t kVersion = 2;
+constexpr int kM = 16;      // rows per group / tile
+constexpr int kK = 32;      // columns per tile
+constexpr int kKComp = 16;  // kept values per tile row
+constexpr int kN = 16;      // channels per panel block (8 coils x Re/Im)
+constexpr int kCoilsPerBlock = 8;
+constexpr int kMaxCoils = 16;
+constexpr int kMaxTilesPerItem = 8;  // work-item split of long groups
+
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define TSN_HIP(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t tsn_status_ = (expr);                                                    \
+    if (tsn_status_ != hipSuccess)                                                      \
+      return ::tsn::fail(TSN_ERR_RUNTIME, "HIP error '%s' at %s:%d",                    \
+                         hipGetErrorString(tsn_status_), __FILE__, __LINE__);           \
+  } while (0)
+
+#define TSN_CHECK(expr)                  \
+  do {                                   \
+    int tsn_rc_ = (expr);                \
+    if (tsn_rc_ != TSN_OK) return tsn_rc_; \
+  } while (0)
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
+
+// One packed direction in host memory (real + imag share the structure).
+struct HostPack {
+  tsn_pack_header h{};
+  std::vector<uint32_t> group_offsets;  // [groups + 1]
+  std::vector<int32_t> group_row_ids;   // [groups * 16]
+  std::vector<int32_t> tile_col_ids;    // [tiles * 32]
+  std::vector<uint16_t> real_comp;      // [tiles * 16 * 16] fp16 bits
+  std::vector<uint32_t> real_meta;      // [tiles * 16]
+  std::vector<uint16_t> imag_comp;
+  std::vector<uint32_t> imag_meta;
+};
+
+int validate_header(const tsn_pack_header& h);
+int validate_pack(const HostPack& p);
+
+// Decode the pair-packed metadata of one tile into 16 row words
+// (reference experiments/packing/prepare_dense_control.py:27-36).
+inline void decode_row_meta(const uint32_t* pair_meta, uint32_t* row_meta) {
+  for (int g = 0; g < 8; ++g) {
+    const uint32_t a = pair_meta[2 * g], b = pair_meta[2 * g + 1];
+    row_meta[g] = (a & 0xFFFFu) | ((b & 0xFFFFu) << 16);
+    row_meta[g + 8] = (a >> 16) | (b & 0xFFFF0000u);
+  }
+}
+
+// Work item of the interpolation kernels: a contiguous tile range of one
+// group.  slot < 0: the item covers the whole group and stores the result;
+// slot >= 0: partial sums go to scratch slot `slot`, combined later.
+struct WorkItem {
+  int32_t group;
+  int32_t tile_begin;
+  int32_t tile_end;
+  int32_t slot;
+};
+struct SplitGroup {
+  int32_t group;
+  int32_t first_slot;
+  int32_t slots;
+  int32_t pad;
+};
+
+}  // namespace tsn
+
+struct tsn_host_pack : tsn::HostPack {};
+
+struct tsn_kb_matrix {
+  uint32_t rows = 0, width = 0;          // samples, numpoints^2
+  uint32_t image0 = 0, image1 = 0, grid0 = 0, grid1 = 0, numpoints = 0;
+  std::vector<int32_t> cols;             // [rows * width] ascending per row
+  std::vector<float> re, im;             // float32 components
+  std::vector<double> coord0, coord1;    // mod(traj*grid + grid/2, grid), float64
+};
+
+struct tsn_direction {
+  tsn_pack_header h{};
+  int device = 0;
+  bool csr = false;
+  // ---- packed 2:4 tiles (device)
+  uint32_t* group_offsets = nullptr;
+  int32_t* group_rows = nullptr;     // [groups * 16] logical output rows, -1 = padding
+  int32_t* tile_cols = nullptr;      // [tiles * 32] panel rows
+  uint4* tile_a = nullptr;           // [tiles * 64] per lane {re 4xf16, im 4xf16}
+  uint16_t* tile_idx = nullptr;      // [tiles * 64] per lane sparse index byte re | im << 8
+  uint4* tile_dense = nullptr;       // [tiles * 64 * 2] per lane {re 8xf16}, {im 8xf16}
+  tsn::WorkItem* items = nullptr;
+  tsn::SplitGroup* splits = nullptr;
+  float* partials = nullptr;         // [slots][2 coil blocks][64 lanes][8]
+  uint32_t n_items = 0, n_splits = 0, n_slots = 0;
+  // ---- FP32 CSR (device)
+  uint32_t csr_rows = 0;
+  uint32_t* row_ptr = nullptr;   // [csr_rows + 1]
+  int32_t* row_ids = nullptr;    // [csr_rows] logical output row
+  int32_t* cols = nullptr;       // [nnz]
+  float2* vals = nullptr;        // [nnz]
+  size_t bytes = 0;
+};

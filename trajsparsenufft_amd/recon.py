@@ -1,0 +1,166 @@
+"""Reconstruction plan: A, A^H, A^H A and CG-SENSE on one MI355X.
+
+Mirrors the reference's solver interface (``launch_adjoint_image``,
+``launch_normal``, ``launch_cg`` and the ``nufft_fp16x2_cg`` CLI,
+``experiments/production/src/nufft_fp16x2_cg_bench.cu:203-298``):
+``cg(measurement, iterations, operator_scale, lam)`` takes the raw
+measurement (torchkbnufft scale), the native operator scale (reference CLI
+argument ``OPERATOR_SCALE`` = raw scale / sqrt(K0*K1)) and the raw Tikhonov
+weight (1e-4).  A status of 0 (non-positive curvature) or a non-finite image
+raises ``TsnError`` with code 3, the reference's exit status 3.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from ._lib import (PREC_DENSE16, PREC_FP32, PREC_HI16, PREC_SPLIT16, TSN_ERR_CG, PlanConfig, TsnError, check,
+                   lib)
+from .device import Direction, _ptr, _require, _stream
+from .pack import TrajectoryPacks, prepare_trajectory
+
+PRECISIONS = {"split16": PREC_SPLIT16, "dense16": PREC_DENSE16, "fp32": PREC_FP32, "hi16": PREC_HI16}
+
+
+@dataclass
+class CgResult:
+    image: torch.Tensor           # complex64 [image0, image1]
+    residuals: torch.Tensor       # float64 [iterations + 1], sqrt(rho)
+    status: torch.Tensor          # int32 [1], 1 = positive curvature throughout
+
+
+class Plan:
+    """Device-resident operator for one trajectory, coil set and precision."""
+
+    def __init__(self, forward: Direction, adjoint: Direction, smaps: torch.Tensor, apod: torch.Tensor,
+                 image_size, grid_size, precision: str = "split16"):
+        self.image_size = tuple(image_size)
+        self.grid_size = tuple(grid_size)
+        self.coils = int(smaps.shape[0])
+        self.precision = precision
+        self.samples = forward.rows
+        _require(smaps, torch.complex64, self.coils * image_size[0] * image_size[1], "smaps")
+        _require(apod, torch.float32, image_size[0] * image_size[1], "apodization")
+        cfg = PlanConfig(image_size[0], image_size[1], grid_size[0], grid_size[1], self.coils, PRECISIONS[precision])
+        out = C.c_void_p()
+        check(lib().tsn_plan_create(C.byref(cfg), forward.handle, adjoint.handle, _ptr(smaps), _ptr(apod), C.byref(out)))
+        self._h = out
+        self._dirs = (forward, adjoint)  # keep the directions alive
+        self._graph_key = None
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().tsn_plan_free(self._h)
+            self._h = None
+
+    @property
+    def device_bytes(self) -> int:
+        return int(lib().tsn_plan_bytes(self._h))
+
+    # ------------------------------------------------------------ operators
+    def _image(self, x: torch.Tensor, name: str) -> None:
+        _require(x, torch.complex64, self.image_size[0] * self.image_size[1], name)
+
+    def forward(self, image: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """A x / sqrt(K0*K1): complex64 [image0, image1] -> [coils, samples]."""
+
+        self._image(image, "image")
+        out = torch.empty((self.coils, self.samples), dtype=torch.complex64, device=image.device) if out is None else out
+        _require(out, torch.complex64, self.coils * self.samples, "samples")
+        check(lib().tsn_forward(self._h, _ptr(image), _ptr(out), _stream()))
+        return out
+
+    def adjoint(self, samples: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        _require(samples, torch.complex64, self.coils * self.samples, "samples")
+        out = torch.empty(self.image_size, dtype=torch.complex64, device=samples.device) if out is None else out
+        self._image(out, "out")
+        check(lib().tsn_adjoint(self._h, _ptr(samples), _ptr(out), _stream()))
+        return out
+
+    def normal(self, image: torch.Tensor, inv_scale_sq: float = 1.0, scaled_lambda: float = 0.0,
+               out: torch.Tensor | None = None) -> torch.Tensor:
+        self._image(image, "image")
+        out = torch.empty_like(image) if out is None else out
+        self._image(out, "out")
+        check(lib().tsn_normal(self._h, _ptr(image), _ptr(out), float(inv_scale_sq), float(scaled_lambda), _stream()))
+        return out
+
+    def operator_scale(self, measurement: torch.Tensor) -> float:
+        """Truth-free native operator scale sqrt(max |A^H (y / sqrt(K0 K1))|).
+
+        With it the CG right-hand side has unit peak magnitude, which keeps the
+        FP16 panels of the split paths in range (the reference calibrates the
+        same quantity offline, ``run_cg_quality_matrix.py:15-25, 82-84``)."""
+
+        k = float(np.sqrt(self.grid_size[0] * self.grid_size[1]))
+        b = self.adjoint(measurement / k)
+        peak = float(torch.max(torch.abs(b)).item())
+        if not np.isfinite(peak) or peak <= 0.0:
+            raise TsnError(TSN_ERR_CG, f"adjoint of the measurement has peak {peak}")
+        return float(np.sqrt(peak))
+
+    # ------------------------------------------------------------ CG
+    def _cg_buffers(self, measurement, iterations, out):
+        _require(measurement, torch.complex64, self.coils * self.samples, "measurement")
+        dev = measurement.device
+        if out is None:
+            out = CgResult(torch.empty(self.image_size, dtype=torch.complex64, device=dev),
+                           torch.empty(iterations + 1, dtype=torch.float64, device=dev),
+                           torch.empty(1, dtype=torch.int32, device=dev))
+        self._image(out.image, "x")
+        _require(out.residuals, torch.float64, iterations + 1, "residuals")
+        _require(out.status, torch.int32, 1, "status")
+        return out
+
+    def cg(self, measurement: torch.Tensor, iterations: int = 10, operator_scale: float = 1.0, lam: float = 1e-4,
+           out: CgResult | None = None) -> CgResult:
+        """Enqueue CG-SENSE on the current stream (no host synchronisation)."""
+
+        out = self._cg_buffers(measurement, iterations, out)
+        check(lib().tsn_cg(self._h, _ptr(measurement), iterations, float(operator_scale), float(lam), _ptr(out.image),
+                           _ptr(out.residuals), _ptr(out.status), _stream()))
+        return out
+
+    def capture_cg(self, measurement: torch.Tensor, iterations: int = 10, operator_scale: float = 1.0,
+                   lam: float = 1e-4, out: CgResult | None = None) -> CgResult:
+        """Capture the whole solve into one hipGraph bound to these buffers."""
+
+        out = self._cg_buffers(measurement, iterations, out)
+        stream = torch.cuda.current_stream()
+        if stream.cuda_stream == 0:
+            raise ValueError("graph capture needs a non-default stream (use torch.cuda.stream(...))")
+        check(lib().tsn_cg_graph_capture(self._h, _ptr(measurement), iterations, float(operator_scale), float(lam),
+                                         _ptr(out.image), _ptr(out.residuals), _ptr(out.status), _stream()))
+        self._graph_key = (measurement.data_ptr(), out.image.data_ptr())
+        return out
+
+    def replay(self) -> None:
+        check(lib().tsn_cg_graph_launch(self._h, _stream()))
+
+
+def check_cg(result: CgResult) -> None:
+    """Host-side failure check after a solve (reference exit status 3)."""
+
+    status = int(result.status.item())
+    nonfinite = int((~torch.isfinite(torch.view_as_real(result.image))).sum().item())
+    if status != 1 or nonfinite:
+        raise TsnError(TSN_ERR_CG, f"CG correctness failure curvature={status} nonfinite={nonfinite}")
+
+
+def build_plan(packs: TrajectoryPacks, smaps: torch.Tensor, precision: str = "split16") -> Plan:
+    """Upload the directions a precision needs and create the plan."""
+
+    image_size, grid_size = packs.kb.image_size, packs.kb.grid_size
+    if precision == "fp32":
+        fwd, adj = Direction.csr(packs.kb, "G"), Direction.csr(packs.kb, "G_T")
+    else:
+        fwd, adj = Direction.upload(packs.forward), Direction.upload(packs.adjoint)
+    apod = torch.from_numpy(np.ascontiguousarray(packs.apod)).to(smaps.device)
+    return Plan(fwd, adj, smaps, apod, image_size, grid_size, precision)
+
+
+__all__ = ["Plan", "CgResult", "build_plan", "check_cg", "prepare_trajectory", "PRECISIONS"]
