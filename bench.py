@@ -1,0 +1,270 @@
+#!/usr/bin/env python3
+"""CG-SENSE reconstructions per second on one MI355X (gfx950).
+
+Workload (the headline configuration): 256x256 image on a 512x512 grid
+(oversampling 2), 8 coils, golden-angle radial trajectory of 402 spokes x 512
+samples (205,824 samples per coil), Kaiser-Bessel width 6, 10 CG iterations,
+lambda = 1e-4.  Synthetic data: a seeded 12-ellipse phantom with smooth phase,
+Gaussian coil maps normalised to sum |S|^2 = 1, measurements from the exact
+FP64 non-uniform DFT of the phantom plus complex Gaussian noise at 40 dB SNR.
+The trajectory is packed once (native packer) and excluded from timing, as are
+uploads.  A "step" is one complete device-resident reconstruction (RHS +
+10 CG iterations) of the split-FP16 2:4-sparse-MFMA path, replayed as one
+hipGraph; eager and the plain FP32 / dense-MFMA paths are timed beside it.
+
+NRMSE figures: magnitude NRMSE against an FP64 CG on the exact NUDFT of the
+same data (evaluated with torch float64 on the GPU; it is only the accuracy
+reference, never timed) and against the plain FP32 reconstruction.
+
+Roofline: the interpolation kernels (one complex interpolation application,
+hi + residual, one direction) are launched on their own and timed with events
+on the launching stream; algorithmic bytes per launch
+B = nnz*8 + 2*n_in*2C*2 + n_out*C*8 (nnz = 36*M).
+
+CPU baseline: the oracle's FP32 CSR + FFT CG-SENSE (oracle/cpu_recon.py) on one
+host thread, one full reconstruction (or a bounded number of CG iterations,
+extrapolated), on rank 0 only.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+IMAGE, GRID, COILS, SPOKES, READOUT, CG_ITERS, LAMBDA, SNR_DB = 256, 512, 8, 402, 512, 10, 1e-4, 40.0
+HBM_PEAK_GBS = 8000.0
+
+
+def parse_args():
+    p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-fp64-reference", action="store_true")
+    p.add_argument("--cpu-budget-s", type=float, default=25.0, help="bound on the CPU baseline sample")
+    p.add_argument("--variants", default="split16,fp32,dense16,hi16")
+    return p.parse_args()
+
+
+class Fp64Nudft:
+    """Exact NUDFT with SENSE in torch float64 on the GPU (data synthesis and
+    the FP64 CG accuracy reference; not part of the measured path)."""
+
+    def __init__(self, torch, traj, smaps, n):
+        self.t = torch
+        om = torch.from_numpy((traj.T * 2.0 * np.pi).astype(np.float32)).double().cuda()
+        k = torch.arange(n, dtype=torch.float64, device="cuda") - n // 2
+        self.e0 = torch.exp(-1j * om[0][:, None] * k[None, :])  # [M, n]
+        self.e1 = torch.exp(-1j * om[1][:, None] * k[None, :])
+        self.s = torch.from_numpy(smaps).to(torch.complex128).cuda()
+
+    def forward(self, x):
+        out = []
+        for c in range(self.s.shape[0]):
+            t = (self.s[c] * x) @ self.e1.T                      # [n0, M]
+            out.append(self.t.sum(self.e0 * t.T, dim=1))
+        return self.t.stack(out)
+
+    def adjoint(self, y):
+        acc = self.t.zeros_like(self.s[0])
+        for c in range(self.s.shape[0]):
+            acc += self.s[c].conj() * (self.e0.conj().T @ (y[c][:, None] * self.e1.conj()))
+        return acc
+
+    def cg(self, y, iterations, lam, grid):
+        """(E^H E + lam) x = E^H y in the native normalisation, reference CG order."""
+
+        t = self.t
+        s = float(grid)  # sqrt(K0*K1) for a square grid
+        b = self.adjoint(y / s) / s
+        lam_n = lam / (s * s)
+        x = t.zeros_like(b)
+        r = b.clone()
+        p = r.clone()
+        rs = t.vdot(r.flatten(), r.flatten()).real
+        for _ in range(iterations):
+            ap = self.adjoint(self.forward(p) / s) / s + lam_n * p
+            alpha = rs / t.vdot(p.flatten(), ap.flatten()).real
+            x = x + alpha * p
+            r = r - alpha * ap
+            rs_new = t.vdot(r.flatten(), r.flatten()).real
+            p = r + (rs_new / rs) * p
+            rs = rs_new
+        return x
+
+
+def magnitude_nrmse(a, b) -> float:
+    a, b = np.abs(np.asarray(a)), np.abs(np.asarray(b))
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def time_launches(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        fn()
+    stop.record()
+    stop.synchronize()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps  # ms
+
+
+def main() -> int:
+    args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
+              "independent slices/frames shard across GPUs without communication", file=sys.stderr)
+        return 2
+    import torch
+    from trajsparsenufft_amd import device, pack, synth
+    from trajsparsenufft_amd.recon import build_plan, check_cg
+
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    torch.manual_seed(0)
+    traj = synth.radial(SPOKES, READOUT, golden=True)
+    x_true = synth.phantom(IMAGE, seed=0)
+    smaps = synth.coil_maps(COILS, IMAGE)
+    ref = Fp64Nudft(torch, traj, smaps, IMAGE)
+    y_clean = ref.forward(torch.from_numpy(x_true).to(torch.complex128).cuda()).cpu().numpy()
+    y = synth.add_noise(y_clean, SNR_DB, seed=1)
+    log(f"[bench] data: {y.shape} samples, SNR {SNR_DB} dB")
+
+    t0 = time.perf_counter()
+    packs = pack.prepare_trajectory(traj, (IMAGE, IMAGE), (GRID, GRID))
+    prep_s = time.perf_counter() - t0
+    fh, ah = packs.forward.header, packs.adjoint.header
+    log(f"[bench] packed in {prep_s:.2f}s: G {fh['tiles']} tiles / {fh['groups']} groups, "
+        f"G_T {ah['tiles']} tiles / {ah['groups']} groups")
+
+    y_dev = torch.from_numpy(y).cuda()
+    smaps_dev = torch.from_numpy(smaps).cuda()
+    variants = [v for v in args.variants.split(",") if v]
+    plans = {v: build_plan(packs, smaps_dev, v) for v in variants}
+    scale = plans["fp32" if "fp32" in plans else variants[0]].operator_scale(y_dev)
+    log(f"[bench] operator scale (native) {scale:.6g}")
+
+    images, timings = {}, {}
+    for v, plan in plans.items():
+        res = plan.cg(y_dev, CG_ITERS, scale, LAMBDA)
+        torch.cuda.synchronize()
+        check_cg(res)
+        images[v] = res.image.cpu().numpy()
+        eager_ms = time_launches(torch, lambda: plan.cg(y_dev, CG_ITERS, scale, LAMBDA, out=res),
+                                 args.steps, args.warmup)
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            gout = plan.capture_cg(y_dev, CG_ITERS, scale, LAMBDA)
+            graph_ms = time_launches(torch, plan.replay, args.steps, args.warmup)
+            stream.synchronize()
+        if not np.array_equal(gout.image.cpu().numpy().view(np.uint32), images[v].view(np.uint32)):
+            raise RuntimeError(f"{v}: graph replay differs from eager")
+        timings[v] = {"eager_ms": eager_ms, "graph_ms": graph_ms}
+        log(f"[bench] {v}: eager {eager_ms:.3f} ms, graph {graph_ms:.3f} ms per reconstruction")
+
+    nrmse = {}
+    if not args.no_fp64_reference:
+        x64 = ref.cg(torch.from_numpy(y).to(torch.complex128).cuda(), CG_ITERS, LAMBDA, GRID).cpu().numpy()
+        nrmse = {f"{v}_vs_fp64": magnitude_nrmse(images[v], x64) for v in images}
+        nrmse["fp64_vs_truth"] = magnitude_nrmse(x64, x_true)
+    if "fp32" in images:
+        for v in images:
+            if v != "fp32":
+                nrmse[f"{v}_vs_fp32"] = magnitude_nrmse(images[v], images["fp32"])
+    log(f"[bench] nrmse {nrmse}")
+    del ref
+
+    # ---- roofline of the interpolation kernels (standalone launches)
+    from trajsparsenufft_amd.device import Direction
+    m, g = fh["output_size"], fh["n_cols"]
+    nnz = fh["nnz"]
+    fwd, adj = Direction.upload(packs.forward), Direction.upload(packs.adjoint)
+    rng = np.random.default_rng(5)
+    gp = torch.from_numpy(rng.standard_normal((1, g, 16)).astype(np.float16)).cuda()
+    sp = torch.from_numpy(rng.standard_normal((1, m, 16)).astype(np.float16)).cuda()
+    out_s = torch.empty((COILS, m), dtype=torch.complex64, device="cuda")
+    out_g = torch.zeros((COILS, g), dtype=torch.complex64, device="cuda")
+    k_steps = max(args.steps, 20)
+    fwd_ms = time_launches(torch, lambda: device.interp(fwd, gp, gp, out_s, COILS, False, False, 1.0 / GRID),
+                           k_steps, args.warmup)
+    adj_ms = time_launches(torch, lambda: device.interp(adj, sp, sp, out_g, COILS, True), k_steps, args.warmup)
+    bytes_fwd = nnz * 8 + 2 * g * 2 * COILS * 2 + m * COILS * 8
+    bytes_adj = nnz * 8 + 2 * m * 2 * COILS * 2 + g * COILS * 8
+    fwd_gbs = bytes_fwd / (fwd_ms * 1e-3) / 1e9
+    adj_gbs = bytes_adj / (adj_ms * 1e-3) / 1e9
+    dominant = ("interp_adjoint_G_T", adj_ms, bytes_adj, adj_gbs) if adj_ms >= fwd_ms else \
+        ("interp_forward_G", fwd_ms, bytes_fwd, fwd_gbs)
+    log(f"[bench] interp forward {fwd_ms * 1e3:.1f} us ({fwd_gbs:.0f} GB/s), "
+        f"adjoint {adj_ms * 1e3:.1f} us ({adj_gbs:.0f} GB/s)")
+
+    # ---- CPU baseline (oracle restatement, one host thread)
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle.cpu_recon import CpuSense
+        cols, re_, im_ = packs.kb.arrays()
+        op = CpuSense(cols, re_, im_, smaps, packs.apod, (IMAGE, IMAGE), (GRID, GRID), workers=1)
+        t_apply = time.perf_counter()
+        op.adjoint(op.forward(images["split16"] if "split16" in images else x_true))
+        apply_s = time.perf_counter() - t_apply
+        applies_per_recon = 1 + 2 * CG_ITERS  # RHS adjoint + (forward + adjoint) per iteration
+        iters = CG_ITERS if apply_s * 2 * (CG_ITERS + 1) <= args.cpu_budget_s else \
+            max(1, int(args.cpu_budget_s / (2 * apply_s)) - 1)
+        t_cpu = time.perf_counter()
+        op.cg(y, iters, scale)
+        cpu_s = time.perf_counter() - t_cpu
+        per_recon = cpu_s * applies_per_recon / (1 + 2 * iters)
+        cpu = {"value": 1.0 / per_recon, "unit": "recon/s", "cores": 1, "kind": "port",
+               "sample": f"oracle FP32 CSR + scipy.fft CG-SENSE at the same shape, {iters} CG iterations timed "
+                         f"({cpu_s:.1f} s), scaled to 10 iterations + RHS; host {platform.processor() or platform.machine()}",
+               "seconds_per_recon": per_recon}
+        log(f"[bench] cpu baseline {per_recon:.2f} s per reconstruction ({iters} iterations timed)")
+
+    head = timings["split16"]
+    value = 1000.0 / head["graph_ms"]
+    out = {
+        "metric": "CG-SENSE recon/s 256²/8-coil/10-CG: split-FP16 smfmac vs FP32, NRMSE≤0.5%",
+        "value": value,
+        "unit": "recon/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["graph_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16x2",
+        "data": "synthetic (seeded ellipse phantom, Gaussian coil maps, exact FP64 NUDFT + 40 dB noise)",
+        "config": {"workload": "cg-sense 256x256 image / 512x512 grid / 8 coils / golden-angle radial 402x512 / "
+                               "KB width 6 / 10 CG / lambda 1e-4",
+                   "samples_per_coil": m, "precision": "fp16 hi+res panels x fp16 2:4 tiles (smfmac), fp32 accumulate, "
+                                                         "fp32 FFT, fp64 CG scalars",
+                   "mode": "hipGraph replay of the full reconstruction"},
+        "recon_per_s": {v: {"eager": 1000.0 / t["eager_ms"], "graph": 1000.0 / t["graph_ms"]} for v, t in timings.items()},
+        "latency_ms": timings,
+        "speedup_split16_vs_fp32": (timings["fp32"]["graph_ms"] / head["graph_ms"]) if "fp32" in timings else None,
+        "nrmse": nrmse,
+        "roofline": {"bound": "hbm", "kernel": dominant[0], "achieved": dominant[3], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": dominant[3] / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": dominant[2], "avg_launch_us": dominant[1] * 1e3},
+        "interp": {"forward_us": fwd_ms * 1e3, "forward_GBs": fwd_gbs, "adjoint_us": adj_ms * 1e3,
+                   "adjoint_GBs": adj_gbs, "bytes_forward": bytes_fwd, "bytes_adjoint": bytes_adj,
+                   "pack_tiles": {"G": fh["tiles"], "G_T": ah["tiles"]}},
+        "prep_seconds": prep_s,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

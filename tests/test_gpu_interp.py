@@ -146,3 +146,25 @@ def test_adjointness_of_packed_pair():
     right = np.vdot(xc, ghy.cpu().numpy().T.astype(np.complex128))
     assert abs(left - right) / max(abs(left), abs(right)) <= 1e-5
     _ = rng
+
+
+@pytest.mark.parametrize("coils", [8, 16])
+def test_long_groups_split_and_reduced_in_order(coils):
+    """Dense radial: G_T groups near the k-space centre exceed 16 tiles, so they
+    are split over several waves and reduced by the combine kernel."""
+
+    from oracle.solver import apply_pack
+    from trajsparsenufft_amd import device
+    from trajsparsenufft_amd.device import Direction
+    _, packs = small_case(32, 96)
+    arrays, header = packs.adjoint.arrays(), packs.adjoint.header
+    assert np.diff(arrays["group_offsets"]).max() > 16
+    d = Direction.upload(packs.adjoint)
+    _, hi, res = random_panels(header["n_cols"], coils, seed=21)
+    out = torch.zeros((coils, header["output_size"]), dtype=torch.complex64, device="cuda")
+    device.interp(d, dev(hi), dev(res), out, coils=coils, conjugate=True)
+    ref = apply_pack(arrays, header["output_size"], hi, res, coils, True)
+    assert rel_l2(out.cpu().numpy(), ref) <= TOL
+    out2 = torch.zeros_like(out)
+    device.interp(d, dev(hi), dev(res), out2, coils=coils, conjugate=True)
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), out2.cpu().numpy().view(np.uint32))

@@ -40,8 +40,8 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
 struct InterpArgs {
   const WorkItem* items;
   uint32_t n_items;
-  const SplitGroup* splits;
-  uint32_t n_splits;
+  const WorkItem* long_items;
+  uint32_t n_long;        // multiple of 8 (padding entries have group < 0)
   const int32_t* group_rows;
   const int32_t* tile_cols;
   const uint4* tile_a;
@@ -56,7 +56,6 @@ struct InterpArgs {
   float* out_f32;         // float2 output (coil- or row-major)
   _Float16* out_hi;       // split output panel blocks
   _Float16* out_res;
-  float* partials;
 };
 
 template <int OM, bool CONJ, bool ACC>
@@ -95,8 +94,8 @@ __device__ __forceinline__ void interp_epilogue(const InterpArgs& a, int group, 
 }
 
 template <int KIND, int NB>
-__device__ __forceinline__ void accumulate_tile(const InterpArgs& a, int t, int4 c0, int4 c1,
-                                                uint4 av, uint4 dv0, uint4 dv1, uint32_t w,
+__device__ __forceinline__ void accumulate_tile(const InterpArgs& a, int4 c0, int4 c1, uint4 av,
+                                                uint4 dv0, uint4 dv1, uint32_t w,
                                                 float4_t (&acc_re)[NB], float4_t (&acc_im)[NB],
                                                 int lane) {
   const int n = lane & 15;
@@ -132,24 +131,14 @@ __device__ __forceinline__ void accumulate_tile(const InterpArgs& a, int t, int4
       }
     }
   }
-  (void)t;
 }
 
-// One wave per work item, four items per 256-thread block.
-template <int KIND, int NB, int OM, bool CONJ, bool ACC>
-__global__ __launch_bounds__(256) void interp_tiles_kernel(InterpArgs a) {
-  const int lane = threadIdx.x & 63;
-  const uint32_t item_id = xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
-  if (item_id >= a.n_items) return;
-  const WorkItem it = a.items[item_id];
+// Accumulate tiles first, first + stride, ... < end, fetching the next tile's
+// metadata while the current tile gathers.
+template <int KIND, int NB>
+__device__ __forceinline__ void run_tiles(const InterpArgs& a, int first, int end, int stride,
+                                          float4_t (&acc_re)[NB], float4_t (&acc_im)[NB], int lane) {
   const int q = lane >> 4;
-  float4_t acc_re[NB], acc_im[NB];
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    acc_re[b] = float4_t{0.f, 0.f, 0.f, 0.f};
-    acc_im[b] = float4_t{0.f, 0.f, 0.f, 0.f};
-  }
-  // software pipeline: metadata of tile t+1 is fetched while tile t gathers
   int4 c0 = {0, 0, 0, 0}, c1 = {0, 0, 0, 0};
   uint4 av = {0, 0, 0, 0}, dv0 = {0, 0, 0, 0}, dv1 = {0, 0, 0, 0};
   uint32_t w = 0;
@@ -165,104 +154,145 @@ __global__ __launch_bounds__(256) void interp_tiles_kernel(InterpArgs a) {
       w = a.tile_idx[static_cast<size_t>(t) * 64 + lane];
     }
   };
-  if (it.tile_begin < it.tile_end) fetch(it.tile_begin);
-  for (int t = it.tile_begin; t < it.tile_end; ++t) {
+  if (first < end) fetch(first);
+  for (int t = first; t < end; t += stride) {
     const int4 k0 = c0, k1 = c1;
     const uint4 kav = av, kd0 = dv0, kd1 = dv1;
     const uint32_t kw = w;
-    if (t + 1 < it.tile_end) fetch(t + 1);
-    accumulate_tile<KIND, NB>(a, t, k0, k1, kav, kd0, kd1, kw, acc_re, acc_im, lane);
+    if (t + stride < end) fetch(t + stride);
+    accumulate_tile<KIND, NB>(a, k0, k1, kav, kd0, kd1, kw, acc_re, acc_im, lane);
   }
-  if (it.slot >= 0) {
+}
+
+// Blocks of kTileWaves waves.  The first n_long blocks each own one long group
+// (waves stride over its tiles and are summed through LDS in wave order); the
+// other blocks hold one ordinary group per wave, XCD-contiguous.
+template <int KIND, int NB, int OM, bool CONJ, bool ACC>
+__global__ __launch_bounds__(kTileWaves * 64) void interp_tiles_kernel(InterpArgs a) {
+  __shared__ float4_t red[kTileWaves][NB][2][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float4_t acc_re[NB], acc_im[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    acc_re[b] = float4_t{0.f, 0.f, 0.f, 0.f};
+    acc_im[b] = float4_t{0.f, 0.f, 0.f, 0.f};
+  }
+  if (blockIdx.x < a.n_long) {
+    const WorkItem it = a.long_items[blockIdx.x];
+    if (it.group < 0) return;  // padding block: uniform exit before the barrier
+    run_tiles<KIND, NB>(a, it.tile_begin + wave, it.tile_end, kTileWaves, acc_re, acc_im, lane);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      float4_t* dst = reinterpret_cast<float4_t*>(
-          a.partials + ((static_cast<size_t>(it.slot) * 2 + b) * 64 + lane) * 8);
-      dst[0] = acc_re[b];
-      dst[1] = acc_im[b];
+      red[wave][b][0][lane] = acc_re[b];
+      red[wave][b][1][lane] = acc_im[b];
+    }
+    __syncthreads();
+    if (wave != 0) return;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      float4_t re = red[0][b][0][lane], im = red[0][b][1][lane];
+      for (int w2 = 1; w2 < kTileWaves; ++w2) {
+        re += red[w2][b][0][lane];
+        im += red[w2][b][1][lane];
+      }
+      interp_epilogue<OM, CONJ, ACC>(a, it.group, b, re, im, lane);
     }
     return;
   }
+  const uint32_t rb = blockIdx.x - a.n_long;
+  const uint32_t item_id = xcd_remap(rb, gridDim.x - a.n_long) * kTileWaves + wave;
+  if (item_id >= a.n_items) return;
+  const WorkItem it = a.items[item_id];
+  run_tiles<KIND, NB>(a, it.tile_begin, it.tile_end, 1, acc_re, acc_im, lane);
 #pragma unroll
   for (int b = 0; b < NB; ++b) interp_epilogue<OM, CONJ, ACC>(a, it.group, b, acc_re[b], acc_im[b], lane);
 }
 
-// Deterministic reduction of split groups: partial slots summed in order.
-template <int NB, int OM, bool CONJ, bool ACC>
-__global__ __launch_bounds__(256) void interp_combine_kernel(InterpArgs a) {
-  const int lane = threadIdx.x & 63;
-  const uint32_t sid = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (sid >= a.n_splits) return;
-  const SplitGroup sg = a.splits[sid];
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    float4_t re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
-    for (int s = 0; s < sg.slots; ++s) {
-      const float4_t* src = reinterpret_cast<const float4_t*>(
-          a.partials + ((static_cast<size_t>(sg.first_slot + s) * 2 + b) * 64 + lane) * 8);
-      re += src[0];
-      im += src[1];
-    }
-    interp_epilogue<OM, CONJ, ACC>(a, sg.group, b, re, im, lane);
+struct CsrArgs {
+  const uint32_t* row_ptr;
+  const int32_t* row_ids;
+  const int32_t* cols;
+  const float2* vals;
+  const int32_t* short_rows;
+  uint32_t n_short;
+  const int32_t* long_rows;
+  uint32_t n_long;  // multiple of 8 (padding entries are -1)
+  const float2* in;
+  uint32_t coils;
+  uint32_t n_out;
+  float scale;
+  float2* out;
+};
+
+// Plain FP32 path: LPR lanes per CSR row (one coil each), FP32 complex
+// coefficients, input rows float2 [row][coils].  Rows longer than
+// kLongRowEntries get a whole block: 256/LPR lane groups stride over the
+// entries and are summed in group order through LDS.
+template <bool CONJ>
+__device__ __forceinline__ void csr_accumulate(const CsrArgs& a, uint32_t k, uint32_t c, bool active,
+                                               float& re, float& im) {
+  const float2 v = a.vals[k];
+  const float2 x = active ? a.in[static_cast<size_t>(a.cols[k]) * a.coils + c] : make_float2(0.f, 0.f);
+  if (CONJ) {
+    re += v.x * x.x + v.y * x.y;
+    im += v.x * x.y - v.y * x.x;
+  } else {
+    re += v.x * x.x - v.y * x.y;
+    im += v.x * x.y + v.y * x.x;
   }
 }
 
-// Plain FP32 path: LPR lanes per CSR row (one coil each), FP32 complex
-// coefficients, input rows float2 [row][coils].
-template <int LPR, int OM, bool CONJ, bool ACC>
-__global__ __launch_bounds__(256) void interp_csr_kernel(
-    const uint32_t* __restrict__ row_ptr, const int32_t* __restrict__ row_ids,
-    const int32_t* __restrict__ cols, const float2* __restrict__ vals, uint32_t rows,
-    const float2* __restrict__ in, uint32_t coils, uint32_t n_out, float scale,
-    float2* __restrict__ out) {
-  const uint32_t gtid = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const uint32_t task = gtid / LPR;
-  const uint32_t c = gtid % LPR;
-  if (task >= rows) return;
-  const uint32_t begin = row_ptr[task], end = row_ptr[task + 1];
-  const bool active = c < coils;
-  float re0 = 0.f, im0 = 0.f, re1 = 0.f, im1 = 0.f;
-  uint32_t k = begin;
-  for (; k + 2 <= end; k += 2) {
-    const int32_t col0 = cols[k], col1 = cols[k + 1];
-    const float2 a0 = vals[k], a1 = vals[k + 1];
-    const float2 x0 = active ? in[static_cast<size_t>(col0) * coils + c] : make_float2(0.f, 0.f);
-    const float2 x1 = active ? in[static_cast<size_t>(col1) * coils + c] : make_float2(0.f, 0.f);
-    if (CONJ) {
-      re0 += a0.x * x0.x + a0.y * x0.y;
-      im0 += a0.x * x0.y - a0.y * x0.x;
-      re1 += a1.x * x1.x + a1.y * x1.y;
-      im1 += a1.x * x1.y - a1.y * x1.x;
-    } else {
-      re0 += a0.x * x0.x - a0.y * x0.y;
-      im0 += a0.x * x0.y + a0.y * x0.x;
-      re1 += a1.x * x1.x - a1.y * x1.y;
-      im1 += a1.x * x1.y + a1.y * x1.x;
-    }
-  }
-  if (k < end) {
-    const float2 a0 = vals[k];
-    const float2 x0 = active ? in[static_cast<size_t>(cols[k]) * coils + c] : make_float2(0.f, 0.f);
-    if (CONJ) {
-      re0 += a0.x * x0.x + a0.y * x0.y;
-      im0 += a0.x * x0.y - a0.y * x0.x;
-    } else {
-      re0 += a0.x * x0.x - a0.y * x0.y;
-      im0 += a0.x * x0.y + a0.y * x0.x;
-    }
-  }
-  if (!active) return;
-  float2 v = make_float2((re0 + re1) * scale, (im0 + im1) * scale);
-  const int32_t row = row_ids[task];
+template <int OM, bool ACC>
+__device__ __forceinline__ void csr_store(const CsrArgs& a, int32_t row, uint32_t c, float re, float im) {
+  float2 v = make_float2(re * a.scale, im * a.scale);
   size_t o;
-  if constexpr (OM == kOutCoilMajor) o = static_cast<size_t>(c) * n_out + row;
-  else o = static_cast<size_t>(row) * coils + c;
+  if constexpr (OM == kOutCoilMajor) o = static_cast<size_t>(c) * a.n_out + row;
+  else o = static_cast<size_t>(row) * a.coils + c;
   if constexpr (ACC) {
-    const float2 prior = out[o];
+    const float2 prior = a.out[o];
     v.x += prior.x;
     v.y += prior.y;
   }
-  out[o] = v;
+  a.out[o] = v;
+}
+
+template <int LPR, int OM, bool CONJ, bool ACC>
+__global__ __launch_bounds__(256) void interp_csr_kernel(CsrArgs a) {
+  constexpr int kGroups = 256 / LPR;
+  __shared__ float2 red[256];
+  const uint32_t seg = threadIdx.x / LPR, c = threadIdx.x % LPR;
+  const bool active = c < a.coils;
+  if (blockIdx.x < a.n_long) {
+    const int32_t task = a.long_rows[blockIdx.x];
+    if (task < 0) return;  // padding block: uniform exit before the barrier
+    float re0 = 0.f, im0 = 0.f;
+    for (uint32_t k = a.row_ptr[task] + seg; k < a.row_ptr[task + 1]; k += kGroups)
+      csr_accumulate<CONJ>(a, k, c, active, re0, im0);
+    red[threadIdx.x] = make_float2(re0, im0);
+    __syncthreads();
+    if (seg != 0 || !active) return;
+    float re = 0.f, im = 0.f;
+    for (int g = 0; g < kGroups; ++g) {
+      re += red[g * LPR + c].x;
+      im += red[g * LPR + c].y;
+    }
+    csr_store<OM, ACC>(a, a.row_ids[task], c, re, im);
+    return;
+  }
+  const uint32_t rb = blockIdx.x - a.n_long;
+  const uint32_t idx = xcd_remap(rb, gridDim.x - a.n_long) * kGroups + seg;
+  if (idx >= a.n_short) return;
+  const int32_t task = a.short_rows[idx];
+  const uint32_t begin = a.row_ptr[task], end = a.row_ptr[task + 1];
+  float re0 = 0.f, im0 = 0.f, re1 = 0.f, im1 = 0.f;
+  uint32_t k = begin;
+  for (; k + 2 <= end; k += 2) {
+    csr_accumulate<CONJ>(a, k, c, active, re0, im0);
+    csr_accumulate<CONJ>(a, k + 1, c, active, re1, im1);
+  }
+  if (k < end) csr_accumulate<CONJ>(a, k, c, active, re0, im0);
+  if (!active) return;
+  csr_store<OM, ACC>(a, a.row_ids[task], c, re0 + re1, im0 + im1);
 }
 
 // hi = RN16(v), res = RN16(v - hi), v = in * scale; in float2 [coil][rows];
@@ -293,14 +323,11 @@ __global__ __launch_bounds__(256) void split_panel_kernel(const float2* __restri
 
 template <int KIND, int NB, int OM>
 static void launch_tiles_om(const InterpArgs& a, bool conj, bool acc, hipStream_t s) {
-  const uint32_t blocks = div_up(a.n_items, 4);
-  const uint32_t cblocks = div_up(a.n_splits, 4);
+  const uint32_t blocks = a.n_long + div_up(a.n_items, kTileWaves);
 #define TSN_LAUNCH(C, A)                                                                     \
   do {                                                                                       \
     if (blocks) hipLaunchKernelGGL((interp_tiles_kernel<KIND, NB, OM, C, A>), dim3(blocks),  \
-                                   dim3(256), 0, s, a);                                      \
-    if (cblocks) hipLaunchKernelGGL((interp_combine_kernel<NB, OM, C, A>), dim3(cblocks),    \
-                                    dim3(256), 0, s, a);                                     \
+                                   dim3(kTileWaves * 64), 0, s, a);                          \
   } while (0)
   if (conj) {
     if (acc) TSN_LAUNCH(true, true); else TSN_LAUNCH(true, false);
@@ -340,8 +367,8 @@ int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* 
   InterpArgs a{};
   a.items = d->items;
   a.n_items = d->n_items;
-  a.splits = d->splits;
-  a.n_splits = d->n_splits;
+  a.long_items = d->long_items;
+  a.n_long = d->n_long;
   a.group_rows = d->group_rows;
   a.tile_cols = d->tile_cols;
   a.tile_a = d->tile_a;
@@ -356,7 +383,6 @@ int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* 
   a.out_f32 = out_f32;
   a.out_hi = out_hi;
   a.out_res = out_res;
-  a.partials = d->partials;
   if (kind == kSparseSplit) launch_tiles<kSparseSplit>(a, out_mode, conj, accumulate, stream);
   else if (kind == kDenseSplit) launch_tiles<kDenseSplit>(a, out_mode, conj, accumulate, stream);
   else launch_tiles<kSparseHi>(a, out_mode, conj, accumulate, stream);
@@ -367,12 +393,11 @@ int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* 
 template <int LPR, int OM>
 static void launch_csr_om(const tsn_direction* d, const float2* in, uint32_t coils, bool conj,
                           bool acc, float scale, float2* out, hipStream_t s) {
-  const uint64_t threads = static_cast<uint64_t>(d->csr_rows) * LPR;
-  const uint32_t blocks = div_up(threads, 256);
-#define TSN_CSR(C, A)                                                                        \
-  hipLaunchKernelGGL((interp_csr_kernel<LPR, OM, C, A>), dim3(blocks), dim3(256), 0, s,      \
-                     d->row_ptr, d->row_ids, d->cols, d->vals, d->csr_rows, in, coils,       \
-                     d->h.output_size, scale, out)
+  CsrArgs a{d->row_ptr, d->row_ids, d->cols, d->vals, d->short_rows, d->n_short, d->long_rows,
+            d->n_long_rows, in, coils, d->h.output_size, scale, out};
+  const uint32_t blocks = d->n_long_rows + div_up(static_cast<uint64_t>(d->n_short) * LPR, 256);
+#define TSN_CSR(C, A) \
+  hipLaunchKernelGGL((interp_csr_kernel<LPR, OM, C, A>), dim3(blocks), dim3(256), 0, s, a)
   if (conj) {
     if (acc) TSN_CSR(true, true); else TSN_CSR(true, false);
   } else {
@@ -431,8 +456,8 @@ static int device_copy(const T* host, size_t count, T** out, size_t* bytes) {
 static void release(tsn_direction* d) {
   if (!d) return;
   void* ptrs[] = {d->group_offsets, d->group_rows, d->tile_cols, d->tile_a, d->tile_idx,
-                  d->tile_dense, d->items, d->splits, d->partials, d->row_ptr, d->row_ids,
-                  d->cols, d->vals};
+                  d->tile_dense, d->items, d->long_items, d->row_ptr, d->row_ids,
+                  d->cols, d->vals, d->short_rows, d->long_rows};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete d;
@@ -481,33 +506,20 @@ static int upload_pack(const HostPack& p, tsn_direction** out) {
       tile_dense[(t * 64 + lane) * 2 + 1] = d1;
     }
   }
-  // work items: long groups split into balanced chunks of <= kMaxTilesPerItem tiles
-  std::vector<WorkItem> items;
-  std::vector<SplitGroup> splits;
-  int32_t slots = 0;
+  // ordinary groups: one wave each; long groups: one block each (padded to a
+  // multiple of 8 blocks so the ordinary blocks keep their XCD mapping)
+  std::vector<WorkItem> items, long_items;
   for (uint32_t g = 0; g < h.groups; ++g) {
     const int32_t b = static_cast<int32_t>(p.group_offsets[g]);
     const int32_t e = static_cast<int32_t>(p.group_offsets[g + 1]);
-    const int32_t n = e - b;
-    if (n <= kMaxTilesPerItem) {
-      items.push_back({static_cast<int32_t>(g), b, e, -1});
-      continue;
-    }
-    const int32_t chunks = (n + kMaxTilesPerItem - 1) / kMaxTilesPerItem;
-    splits.push_back({static_cast<int32_t>(g), slots, chunks, 0});
-    for (int32_t c = 0; c < chunks; ++c) {
-      const int32_t cb = b + static_cast<int32_t>(static_cast<int64_t>(n) * c / chunks);
-      const int32_t ce = b + static_cast<int32_t>(static_cast<int64_t>(n) * (c + 1) / chunks);
-      items.push_back({static_cast<int32_t>(g), cb, ce, slots + c});
-    }
-    slots += chunks;
+    (e - b <= kSplitAboveTiles ? items : long_items).push_back({static_cast<int32_t>(g), b, e, 0});
   }
+  while (long_items.size() % 8) long_items.push_back({-1, 0, 0, 0});
   auto* d = new tsn_direction();
   d->h = h;
   (void)hipGetDevice(&d->device);
   d->n_items = static_cast<uint32_t>(items.size());
-  d->n_splits = static_cast<uint32_t>(splits.size());
-  d->n_slots = static_cast<uint32_t>(slots);
+  d->n_long = static_cast<uint32_t>(long_items.size());
   size_t bytes = 0;
   int rc = TSN_OK;
   if (rc == TSN_OK) rc = device_copy(p.group_offsets.data(), p.group_offsets.size(), &d->group_offsets, &bytes);
@@ -517,13 +529,7 @@ static int upload_pack(const HostPack& p, tsn_direction** out) {
   if (rc == TSN_OK) rc = device_copy(tile_idx.data(), tile_idx.size(), &d->tile_idx, &bytes);
   if (rc == TSN_OK) rc = device_copy(tile_dense.data(), tile_dense.size(), &d->tile_dense, &bytes);
   if (rc == TSN_OK) rc = device_copy(items.data(), items.size(), &d->items, &bytes);
-  if (rc == TSN_OK) rc = device_copy(splits.data(), splits.size(), &d->splits, &bytes);
-  if (rc == TSN_OK && slots > 0) {
-    const size_t count = static_cast<size_t>(slots) * 2 * 64 * 8;
-    const hipError_t e = hipMalloc(reinterpret_cast<void**>(&d->partials), count * sizeof(float));
-    if (e != hipSuccess) rc = fail(TSN_ERR_RUNTIME, "hipMalloc partials: %s", hipGetErrorString(e));
-    else bytes += count * sizeof(float);
-  }
+  if (rc == TSN_OK) rc = device_copy(long_items.data(), long_items.size(), &d->long_items, &bytes);
   if (rc != TSN_OK) {
     release(d);
     return rc;
@@ -607,9 +613,16 @@ int tsn_direction_upload_csr(const tsn_kb_matrix* kb, int view, tsn_direction** 
     h.n_cols = m;
     h.active_rows = static_cast<uint32_t>(rows.size());
   }
+  std::vector<int32_t> short_rows, long_rows;
+  for (uint32_t t = 0; t + 1 < ptr.size(); ++t)
+    (ptr[t + 1] - ptr[t] <= static_cast<uint32_t>(kLongRowEntries) ? short_rows : long_rows)
+        .push_back(static_cast<int32_t>(t));
+  while (long_rows.size() % 8) long_rows.push_back(-1);
   auto* d = new tsn_direction();
   d->h = h;
   d->csr = true;
+  d->n_short = static_cast<uint32_t>(short_rows.size());
+  d->n_long_rows = static_cast<uint32_t>(long_rows.size());
   (void)hipGetDevice(&d->device);
   d->csr_rows = static_cast<uint32_t>(rows.size());
   size_t bytes = 0;
@@ -617,6 +630,8 @@ int tsn_direction_upload_csr(const tsn_kb_matrix* kb, int view, tsn_direction** 
   if (rc == TSN_OK) rc = device_copy(rows.data(), rows.size(), &d->row_ids, &bytes);
   if (rc == TSN_OK) rc = device_copy(cols.data(), cols.size(), &d->cols, &bytes);
   if (rc == TSN_OK) rc = device_copy(vals.data(), vals.size(), &d->vals, &bytes);
+  if (rc == TSN_OK) rc = device_copy(short_rows.data(), short_rows.size(), &d->short_rows, &bytes);
+  if (rc == TSN_OK) rc = device_copy(long_rows.data(), long_rows.size(), &d->long_rows, &bytes);
   if (rc != TSN_OK) {
     release(d);
     return rc;

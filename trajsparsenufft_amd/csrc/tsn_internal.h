@@ -20,7 +20,9 @@ constexpr int kKComp = 16;  // kept values per tile row
 constexpr int kN = 16;      // channels per panel block (8 coils x Re/Im)
 constexpr int kCoilsPerBlock = 8;
 constexpr int kMaxCoils = 16;
-constexpr int kMaxTilesPerItem = 8;  // work-item split of long groups
+constexpr int kTileWaves = 16;        // waves per interpolation block (1024 threads)
+constexpr int kSplitAboveTiles = 16;  // longer groups get a whole block (waves stride over tiles)
+constexpr int kLongRowEntries = 128;  // FP32 CSR rows longer than this get a whole block
 
 void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -66,19 +68,13 @@ inline void decode_row_meta(const uint32_t* pair_meta, uint32_t* row_meta) {
   }
 }
 
-// Work item of the interpolation kernels: a contiguous tile range of one
-// group.  slot < 0: the item covers the whole group and stores the result;
-// slot >= 0: partial sums go to scratch slot `slot`, combined later.
+// Work item of the interpolation kernels: the tile range of one group.
+// Groups longer than kSplitAboveTiles are listed separately and get a whole
+// block whose waves stride over the tiles; group < 0 marks a padding entry.
 struct WorkItem {
   int32_t group;
   int32_t tile_begin;
   int32_t tile_end;
-  int32_t slot;
-};
-struct SplitGroup {
-  int32_t group;
-  int32_t first_slot;
-  int32_t slots;
   int32_t pad;
 };
 
@@ -105,15 +101,17 @@ struct tsn_direction {
   uint4* tile_a = nullptr;           // [tiles * 64] per lane {re 4xf16, im 4xf16}
   uint16_t* tile_idx = nullptr;      // [tiles * 64] per lane sparse index byte re | im << 8
   uint4* tile_dense = nullptr;       // [tiles * 64 * 2] per lane {re 8xf16}, {im 8xf16}
-  tsn::WorkItem* items = nullptr;
-  tsn::SplitGroup* splits = nullptr;
-  float* partials = nullptr;         // [slots][2 coil blocks][64 lanes][8]
-  uint32_t n_items = 0, n_splits = 0, n_slots = 0;
+  tsn::WorkItem* items = nullptr;    // groups with <= kSplitAboveTiles tiles
+  tsn::WorkItem* long_items = nullptr;  // longer groups, padded to a multiple of 8
+  uint32_t n_items = 0, n_long = 0;
   // ---- FP32 CSR (device)
   uint32_t csr_rows = 0;
   uint32_t* row_ptr = nullptr;   // [csr_rows + 1]
   int32_t* row_ids = nullptr;    // [csr_rows] logical output row
   int32_t* cols = nullptr;       // [nnz]
   float2* vals = nullptr;        // [nnz]
+  int32_t* short_rows = nullptr; // CSR rows with <= kLongRowEntries entries
+  int32_t* long_rows = nullptr;  // longer rows, padded with -1 to a multiple of 8
+  uint32_t n_short = 0, n_long_rows = 0;
   size_t bytes = 0;
 };

@@ -68,6 +68,7 @@ class Plan:
     def forward(self, image: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """A x / sqrt(K0*K1): complex64 [image0, image1] -> [coils, samples]."""
 
+        image = image.contiguous()
         self._image(image, "image")
         out = torch.empty((self.coils, self.samples), dtype=torch.complex64, device=image.device) if out is None else out
         _require(out, torch.complex64, self.coils * self.samples, "samples")
@@ -75,6 +76,7 @@ class Plan:
         return out
 
     def adjoint(self, samples: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        samples = samples.contiguous()
         _require(samples, torch.complex64, self.coils * self.samples, "samples")
         out = torch.empty(self.image_size, dtype=torch.complex64, device=samples.device) if out is None else out
         self._image(out, "out")
@@ -83,6 +85,7 @@ class Plan:
 
     def normal(self, image: torch.Tensor, inv_scale_sq: float = 1.0, scaled_lambda: float = 0.0,
                out: torch.Tensor | None = None) -> torch.Tensor:
+        image = image.contiguous()
         self._image(image, "image")
         out = torch.empty_like(image) if out is None else out
         self._image(out, "out")
@@ -105,6 +108,8 @@ class Plan:
 
     # ------------------------------------------------------------ CG
     def _cg_buffers(self, measurement, iterations, out):
+        if not measurement.is_contiguous():
+            raise ValueError("measurement must be contiguous (the solve keeps reading it)")
         _require(measurement, torch.complex64, self.coils * self.samples, "measurement")
         dev = measurement.device
         if out is None:
