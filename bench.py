@@ -51,6 +51,8 @@ def parse_args():
     p.add_argument("--no-fp64-reference", action="store_true")
     p.add_argument("--cpu-budget-s", type=float, default=25.0, help="bound on the CPU baseline sample")
     p.add_argument("--variants", default="split16,fp32,dense16,hi16")
+    p.add_argument("--roofline-only", action="store_true",
+                   help="only the standalone interpolation launches and a 64 MiB copy (counter runs)")
     return p.parse_args()
 
 
@@ -120,12 +122,103 @@ def time_launches(torch, fn, steps, warmup):
     return start.elapsed_time(stop) / steps  # ms
 
 
+TRAFFIC_FILES = {"fetch": "profiles/interp_il_256x512_8coil_radial402_fetch_size.csv",
+                 "write": "profiles/interp_il_256x512_8coil_radial402_write_size.csv"}
+COPY_BYTES = 64 << 20  # calibration copy of --roofline-only
+
+
+def measured_traffic(kernel_substr: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
+    FETCH_SIZE and WRITE_SIZE runs of ``bench.py --roofline-only`` (KiB per
+    dispatch), each counter scaled by what it reads for the 64 MiB
+    device-to-device copy of the same run.  None if the files are absent."""
+
+    import csv
+    root = os.path.dirname(os.path.abspath(__file__))
+    total = 0.0
+    for counter, rel in TRAFFIC_FILES.items():
+        path = os.path.join(root, rel)
+        if not os.path.exists(path):
+            return None
+        kern, copy = [], []
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                v = float(row["Counter_Value"])
+                if kernel_substr in row["Kernel_Name"]:
+                    kern.append(v)
+                elif "copyBuffer" in row["Kernel_Name"] and v > 1024:
+                    copy.append(v)
+        if not kern or not copy:
+            return None
+        factor = COPY_BYTES / (1024.0 * sum(copy) / len(copy))
+        total += 1024.0 * sum(kern) / len(kern) * factor
+    return total
+
+
+def interp_roofline(torch, packs, steps, warmup, log):
+    """Standalone launches of the packed interpolation (interleaved split
+    panel, the layout the reconstruction uses), timed with events on the
+    launch stream; algorithmic bytes per launch as defined for the unit of one
+    complex interpolation application: nnz * (2 + 2 + 4) + 2 * n_in * 2C * 2 +
+    n_out * C * 8."""
+
+    from trajsparsenufft_amd import device
+    from trajsparsenufft_amd.device import Direction
+    fh = packs.forward.header
+    m, g, nnz = fh["output_size"], fh["n_cols"], fh["nnz"]
+    fwd, adj = Direction.upload(packs.forward), Direction.upload(packs.adjoint)
+    rng = np.random.default_rng(5)
+
+    def panel(rows):
+        v = rng.standard_normal((COILS, rows)) + 1j * rng.standard_normal((COILS, rows))
+        return device.split_panel_il(torch.from_numpy(v.astype(np.complex64)).cuda(), COILS)
+
+    gp, sp = panel(g), panel(m)
+    out_s = torch.empty((COILS, m), dtype=torch.complex64, device="cuda")
+    out_g = torch.zeros((COILS, g), dtype=torch.complex64, device="cuda")
+    fwd_ms = time_launches(torch, lambda: device.interp_il(fwd, gp, out_s, COILS, False, False, 1.0 / GRID),
+                           steps, warmup)
+    adj_ms = time_launches(torch, lambda: device.interp_il(adj, sp, out_g, COILS, True), steps, warmup)
+    res = {}
+    for name, ms, n_in, n_out, conj in (("forward", fwd_ms, g, m, "false"), ("adjoint", adj_ms, m, g, "true")):
+        b = nnz * 8 + 2 * n_in * 2 * COILS * 2 + n_out * COILS * 8
+        kernel = f"interp_tiles_kernel<0, 1, true, 0, {conj}, false>"
+        res[name] = {"kernel": kernel, "us": ms * 1e3, "bytes": b, "GBs": b / (ms * 1e-3) / 1e9,
+                     "traffic": measured_traffic(kernel)}
+        log(f"[bench] interp {name} {ms * 1e3:.1f} us ({res[name]['GBs']:.0f} GB/s algorithmic, "
+            f"HBM traffic {res[name]['traffic']})")
+    res["dominant"] = res["adjoint"] if adj_ms >= fwd_ms else res["forward"]
+    res["traffic_source"] = (f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --roofline-only, "
+                             f"scaled by a {COPY_BYTES >> 20} MiB copy: {', '.join(TRAFFIC_FILES.values())}")
+    return res
+
+
+def roofline_only(args) -> int:
+    """Interpolation launches plus the calibration copy, for counter runs."""
+
+    import torch
+    from trajsparsenufft_amd import pack, synth
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    traj = synth.radial(SPOKES, READOUT, golden=True)
+    packs = pack.prepare_trajectory(traj, (IMAGE, IMAGE), (GRID, GRID))
+    roof = interp_roofline(torch, packs, args.steps, args.warmup, log)
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({k: roof[k] for k in ("forward", "adjoint")}), flush=True)
+    return 0
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
               "independent slices/frames shard across GPUs without communication", file=sys.stderr)
         return 2
+    if args.roofline_only:
+        return roofline_only(args)
     import torch
     from trajsparsenufft_amd import device, pack, synth
     from trajsparsenufft_amd.recon import build_plan, check_cg
@@ -185,27 +278,8 @@ def main() -> int:
     del ref
 
     # ---- roofline of the interpolation kernels (standalone launches)
-    from trajsparsenufft_amd.device import Direction
-    m, g = fh["output_size"], fh["n_cols"]
-    nnz = fh["nnz"]
-    fwd, adj = Direction.upload(packs.forward), Direction.upload(packs.adjoint)
-    rng = np.random.default_rng(5)
-    gp = torch.from_numpy(rng.standard_normal((1, g, 16)).astype(np.float16)).cuda()
-    sp = torch.from_numpy(rng.standard_normal((1, m, 16)).astype(np.float16)).cuda()
-    out_s = torch.empty((COILS, m), dtype=torch.complex64, device="cuda")
-    out_g = torch.zeros((COILS, g), dtype=torch.complex64, device="cuda")
-    k_steps = max(args.steps, 20)
-    fwd_ms = time_launches(torch, lambda: device.interp(fwd, gp, gp, out_s, COILS, False, False, 1.0 / GRID),
-                           k_steps, args.warmup)
-    adj_ms = time_launches(torch, lambda: device.interp(adj, sp, sp, out_g, COILS, True), k_steps, args.warmup)
-    bytes_fwd = nnz * 8 + 2 * g * 2 * COILS * 2 + m * COILS * 8
-    bytes_adj = nnz * 8 + 2 * m * 2 * COILS * 2 + g * COILS * 8
-    fwd_gbs = bytes_fwd / (fwd_ms * 1e-3) / 1e9
-    adj_gbs = bytes_adj / (adj_ms * 1e-3) / 1e9
-    dominant = ("interp_adjoint_G_T", adj_ms, bytes_adj, adj_gbs) if adj_ms >= fwd_ms else \
-        ("interp_forward_G", fwd_ms, bytes_fwd, fwd_gbs)
-    log(f"[bench] interp forward {fwd_ms * 1e3:.1f} us ({fwd_gbs:.0f} GB/s), "
-        f"adjoint {adj_ms * 1e3:.1f} us ({adj_gbs:.0f} GB/s)")
+    roof = interp_roofline(torch, packs, max(args.steps, 20), args.warmup, log)
+    dominant = roof["dominant"]
 
     # ---- CPU baseline (oracle restatement, one host thread)
     cpu = None
@@ -253,12 +327,12 @@ def main() -> int:
         "latency_ms": timings,
         "speedup_split16_vs_fp32": (timings["fp32"]["graph_ms"] / head["graph_ms"]) if "fp32" in timings else None,
         "nrmse": nrmse,
-        "roofline": {"bound": "hbm", "kernel": dominant[0], "achieved": dominant[3], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dominant[3] / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes": dominant[2], "avg_launch_us": dominant[1] * 1e3},
-        "interp": {"forward_us": fwd_ms * 1e3, "forward_GBs": fwd_gbs, "adjoint_us": adj_ms * 1e3,
-                   "adjoint_GBs": adj_gbs, "bytes_forward": bytes_fwd, "bytes_adjoint": bytes_adj,
-                   "pack_tiles": {"G": fh["tiles"], "G_T": ah["tiles"]}},
+        "roofline": {"bound": "hbm", "kernel": dominant["kernel"], "achieved": dominant["GBs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dominant["GBs"] / HBM_PEAK_GBS,
+                     "traffic": dominant["traffic"], "algorithmic_bytes": dominant["bytes"],
+                     "avg_launch_us": dominant["us"], "traffic_source": roof["traffic_source"]},
+        "interp": {k: roof[k] for k in ("forward", "adjoint")} | {
+            "pack_tiles": {"G": fh["tiles"], "G_T": ah["tiles"]}},
         "prep_seconds": prep_s,
         "cpu_baseline": cpu,
     }

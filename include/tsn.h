@@ -133,6 +133,13 @@ int tsn_direction_load(const char* real_dir, const char* imag_dir, tsn_direction
 /* FP32 CSR direction of the plain path: view 0 = G, 1 = G_T (exact transpose). */
 int tsn_direction_upload_csr(const tsn_kb_matrix* kb, int view, tsn_direction** out);
 int tsn_direction_header(const tsn_direction* dir, tsn_pack_header* out);
+/* Re-cut the work schedule of a packed direction: regular units are runs of
+ * whole groups holding at most unit_tiles tiles each (0 = default, one round
+ * of resident waves).  Synchronizes the device; no counterpart in the
+ * reference (its launch maps one group per warp), used by the autotuner. */
+int tsn_direction_schedule(tsn_direction* dir, uint32_t unit_tiles);
+/* Tile budget of the current schedule (0 for CSR directions). */
+uint32_t tsn_direction_unit_tiles(const tsn_direction* dir);
 /* Device bytes the direction occupies (pack + index arrays). */
 size_t tsn_direction_bytes(const tsn_direction* dir);
 void tsn_direction_free(tsn_direction* dir);
@@ -157,6 +164,16 @@ int tsn_interp_f32(const tsn_direction* csr, const void* in, void* out, uint32_t
  * blocks (reference integrated_complex_to_split2_half_rows, :339-359). */
 int tsn_split_panel(const void* in, uint32_t rows, uint32_t coils, float scale, void* hi, void* res,
                     void* stream);
+/* Interleaved split panel: the layout plans use internally (no counterpart in
+ * the reference, which keeps hi and residual in two panels).  uint32 per
+ * channel = fp16 hi | fp16 residual << 16, blocks of [rows][16] for
+ * ceil(coils/8) coil blocks; one 64-byte gather per column instead of two.
+ * tsn_split_panel_il rounds exactly as tsn_split_panel; tsn_interp_il is
+ * tsn_interp on such a panel (same tiles, same sums, same output). */
+int tsn_split_panel_il(const void* in, uint32_t rows, uint32_t coils, float scale, void* panel,
+                       void* stream);
+int tsn_interp_il(const tsn_direction* dir, const void* panel, void* out, uint32_t coils,
+                  int conjugate, int accumulate, float scale, int out_layout, void* stream);
 
 /* ------------------------------------------------------------------ plan */
 

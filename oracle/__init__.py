@@ -16,7 +16,7 @@ Modules
              split24, compression/metadata, header, dense expansion.
 ``solver``   Component-wise arithmetic model (FP16 matrix x split2 panels),
              FP32 CSR operator and the reference CG update order.
-``synth``    Seeded synthetic inputs (trajectories, phantom, coil maps, noise).
+``cpu_recon`` FP32 CSR + scipy.fft CG-SENSE, the timed CPU baseline.
 
 Parity status of each function is stated in its docstring ("pinned by ...",
 or "parity unpinned").

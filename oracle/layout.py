@@ -3,7 +3,7 @@
 TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
 
 Restates, function by function, the reference's structural CPU oracle and
-exporters so the native packer (``trajsparsenufft_amd/csrc/tsn_packer.cpp``)
+exporters so the native packer (``trajsparsenufft_amd/csrc/packer.cpp``)
 can be checked bit for bit:
 
 * ``src/nufft_layout_oracle.py``: ``morton2d`` (:110-121),

@@ -168,3 +168,49 @@ def test_long_groups_split_and_reduced_in_order(coils):
     out2 = torch.zeros_like(out)
     device.interp(d, dev(hi), dev(res), out2, coils=coils, conjugate=True)
     assert np.array_equal(out.cpu().numpy().view(np.uint32), out2.cpu().numpy().view(np.uint32))
+
+
+@pytest.mark.parametrize("view", ["G", "G_T"])
+def test_schedule_cut_is_bitwise_invariant(view):
+    """Re-cutting the work units (runs of whole groups per wave) changes only
+    which wave streams a group, never the order of its tile sums."""
+
+    from trajsparsenufft_amd import device
+    d, _, header = _direction(view)
+    _, hi, res = random_panels(header["n_cols"], 12, seed=31)
+    h, r = dev(hi), dev(res)
+    outs = []
+    for unit in (0, 1, 3, 64):
+        d.schedule(unit)
+        assert unit == 0 or d.unit_tiles == unit
+        out = torch.zeros((12, header["output_size"]), dtype=torch.complex64, device="cuda")
+        device.interp(d, h, r, out, coils=12, conjugate=view == "G_T")
+        outs.append(out.cpu().numpy().view(np.uint32))
+    d.schedule(0)
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+
+
+@pytest.mark.parametrize("view", ["G", "G_T"])
+@pytest.mark.parametrize("coils", [8, 12])
+def test_interleaved_panel_path_is_bitwise_the_split_path(view, coils):
+    """tsn_split_panel_il rounds exactly like tsn_split_panel, and
+    tsn_interp_il sums the same products in the same order as tsn_interp."""
+
+    from trajsparsenufft_amd import device
+    d, _, header = _direction(view)
+    rng = np.random.default_rng(41)
+    v = torch.from_numpy((rng.standard_normal((coils, header["n_cols"]))
+                          + 1j * rng.standard_normal((coils, header["n_cols"]))).astype(np.complex64)).cuda()
+    hi, res = device.split_panel(v, coils, scale=0.5)
+    il = device.split_panel_il(v, coils, scale=0.5)
+    ilw = il.cpu().numpy().view(np.uint32)
+    assert np.array_equal(ilw & 0xFFFF, hi.cpu().numpy().view(np.uint16))
+    assert np.array_equal(ilw >> 16, res.cpu().numpy().view(np.uint16))
+    conj = view == "G_T"
+    out = torch.zeros((coils, header["output_size"]), dtype=torch.complex64, device="cuda")
+    out_il = torch.zeros_like(out)
+    device.interp(d, hi, res, out, coils=coils, conjugate=conj, scale=0.25)
+    device.interp_il(d, il, out_il, coils=coils, conjugate=conj, scale=0.25)
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), out_il.cpu().numpy().view(np.uint32))
+    device.interp_il(d, il, out_il, coils=coils, conjugate=conj, scale=0.25, accumulate=True)
+    np.testing.assert_allclose(out_il.cpu().numpy(), 2 * out.cpu().numpy(), rtol=1e-6, atol=1e-6)

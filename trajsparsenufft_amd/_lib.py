@@ -62,11 +62,15 @@ SIGNATURES = {
     "tsn_direction_upload_csr": (C.c_int, [P, C.c_int, PP]),
     "tsn_direction_header": (C.c_int, [P, C.POINTER(PackHeader)]),
     "tsn_direction_bytes": (C.c_size_t, [P]),
+    "tsn_direction_schedule": (C.c_int, [P, C.c_uint32]),
+    "tsn_direction_unit_tiles": (C.c_uint32, [P]),
     "tsn_direction_free": (None, [P]),
     "tsn_interp": (C.c_int, [P, P, P, P, U32, C.c_int, C.c_int, C.c_float, C.c_int, P]),
     "tsn_interp_dense": (C.c_int, [P, P, P, P, U32, C.c_int, C.c_int, C.c_float, C.c_int, P]),
     "tsn_interp_f32": (C.c_int, [P, P, P, U32, C.c_int, C.c_int, C.c_float, C.c_int, P]),
     "tsn_split_panel": (C.c_int, [P, U32, U32, C.c_float, P, P, P]),
+    "tsn_split_panel_il": (C.c_int, [P, U32, U32, C.c_float, P, P]),
+    "tsn_interp_il": (C.c_int, [P, P, P, U32, C.c_int, C.c_int, C.c_float, C.c_int, P]),
     "tsn_plan_create": (C.c_int, [C.POINTER(PlanConfig), P, P, P, P, PP]),
     "tsn_plan_free": (None, [P]),
     "tsn_plan_bytes": (C.c_size_t, [P]),

@@ -6,8 +6,8 @@
 // workgroup of eight waves transforms eight coils of one image column or grid
 // row, so every global read and write is a contiguous block of rows with all
 // coils interleaved.  Fused: SENSE expand x*S_c, apodization, zero-pad (forward
-// first axis), FP16 hi/residual split into the reference's N16 panel layout
-// (forward second axis), crop (inverse, pruned last radix-8 stage), SENSE
+// first axis), FP16 hi/residual split into N16 panel rows holding hi | res << 16
+// per channel (forward second axis), crop (inverse, pruned last radix-8 stage), SENSE
 // combine sum_c conj(S_c) img_c, operator scaling, Tikhonov term and the CG dot
 // partial <p, Ap> (inverse second axis).
 //
@@ -99,24 +99,21 @@ __global__ __launch_bounds__(512) void fwd_axis1_kernel(FftArgs a) {
     const uint64_t r = static_cast<uint64_t>(k0) * K + k1;
     const float2* src = lds + k1 * 8;
     if constexpr (OUT == 0) {
-      half8 hre, him, rre, rim;  // channels 0..7 = Re coils, 8..15 = Im coils
+      uint32_t w[16];  // channel c = Re coil c, c + 8 = Im coil c; hi | res << 16
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
         const float2 v = src[c];
         _Float16 h, r2;
         split16(v.x, h, r2);
-        hre[c] = h;
-        rre[c] = r2;
+        w[c] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
+               (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r2)) << 16);
         split16(v.y, h, r2);
-        him[c] = h;
-        rim[c] = r2;
+        w[c + 8] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
+                   (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r2)) << 16);
       }
-      uint4* dh = reinterpret_cast<uint4*>(a.panel_hi + (chunk * grid_rows + r) * 16);
-      uint4* dr = reinterpret_cast<uint4*>(a.panel_res + (chunk * grid_rows + r) * 16);
-      dh[0] = __builtin_bit_cast(uint4, hre);
-      dh[1] = __builtin_bit_cast(uint4, him);
-      dr[0] = __builtin_bit_cast(uint4, rre);
-      dr[1] = __builtin_bit_cast(uint4, rim);
+      uint4* d4 = reinterpret_cast<uint4*>(a.panel_il + (chunk * grid_rows + r) * 16);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) d4[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
     } else {
       float2* dst = a.panel32 + r * a.coils + chunk * 8;
       for (uint32_t i = 0; i < cw; ++i) dst[i] = src[i];

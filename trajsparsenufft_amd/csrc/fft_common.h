@@ -18,8 +18,7 @@ struct FftArgs {
   const float2* smaps_t;   // [n1][coil][n0]
   const float* apod_t;     // [n1][n0]
   float2* t_fwd;           // [K0][n1][coil]
-  _Float16* panel_hi;      // [block][K0*K1][16]
-  _Float16* panel_res;
+  uint32_t* panel_il;      // [block][K0*K1][16] hi | res << 16
   float2* panel32;         // [K0*K1][coil]
   const float2* grid;      // [K0*K1][coil]
   float2* t_inv;           // [n1][K0][coil]

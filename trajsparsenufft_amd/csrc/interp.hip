@@ -27,7 +27,7 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 
 namespace tsn {
 
-enum OutMode { kOutCoilMajor = 0, kOutRowMajor = 1, kOutSplit16 = 2 };
+enum OutMode { kOutCoilMajor = 0, kOutRowMajor = 1, kOutSplit16 = 2, kOutSplit16IL = 3 };
 enum TileKind { kSparseSplit = 0, kDenseSplit = 1, kSparseHi = 2 };
 
 // Block b runs on XCD b % 8; give each XCD a contiguous range of items.
@@ -43,27 +43,30 @@ struct InterpArgs {
   const WorkItem* long_items;
   uint32_t n_long;        // multiple of 8 (padding entries have group < 0)
   const int32_t* group_rows;
-  const int32_t* tile_cols;
+  const uint32_t* group_offsets;
+  const uint32_t* tile_cols;  // column * 32 | last-tile flag (bit 31 of word 2 of each lane segment)
+  uint32_t n_tiles, n_groups;
   const uint4* tile_a;
   const uint16_t* tile_idx;
   const uint4* tile_dense;
   const _Float16* panel_hi;
   const _Float16* panel_res;
-  uint64_t panel_stride;  // halves between coil blocks of the input panel
+  const uint32_t* panel_il;  // interleaved hi|res panel (plan-internal layout)
+  uint64_t panel_stride;     // elements between coil blocks of the input panel
   uint32_t coils;
   uint32_t n_out;         // output rows (for layouts)
   float scale;
   float* out_f32;         // float2 output (coil- or row-major)
   _Float16* out_hi;       // split output panel blocks
   _Float16* out_res;
+  uint32_t* out_il;       // interleaved split output panel blocks
 };
 
 template <int OM, bool CONJ, bool ACC>
-__device__ __forceinline__ void interp_epilogue(const InterpArgs& a, int group, int b,
+__device__ __forceinline__ void interp_epilogue(const InterpArgs& a, const int4& rows, int b,
                                                 const float4_t& acc_re, const float4_t& acc_im,
                                                 int lane) {
-  const int n = lane & 15, q = lane >> 4;
-  const int4 rows = *reinterpret_cast<const int4*>(a.group_rows + group * 16 + q * 4);
+  const int n = lane & 15;
   const int rr[4] = {rows.x, rows.y, rows.z, rows.w};
   const uint32_t coil = b * kCoilsPerBlock + (n & 7);
   const int part = n >> 3;
@@ -76,12 +79,17 @@ __device__ __forceinline__ void interp_epilogue(const InterpArgs& a, int group, 
     v *= a.scale;
     const int row = rr[i];
     if (row < 0) continue;
-    if constexpr (OM == kOutSplit16) {
+    if constexpr (OM == kOutSplit16 || OM == kOutSplit16IL) {
       const _Float16 hi = static_cast<_Float16>(v);
       const _Float16 res = static_cast<_Float16>(v - static_cast<float>(hi));
       const size_t o = (static_cast<size_t>(b) * a.n_out + row) * kN + n;
-      a.out_hi[o] = hi;
-      a.out_res[o] = res;
+      if constexpr (OM == kOutSplit16IL) {
+        a.out_il[o] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, hi)) |
+                      (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, res)) << 16);
+      } else {
+        a.out_hi[o] = hi;
+        a.out_res[o] = res;
+      }
     } else {
       if (coil >= a.coils) continue;
       size_t o;
@@ -93,84 +101,248 @@ __device__ __forceinline__ void interp_epilogue(const InterpArgs& a, int group, 
   }
 }
 
-template <int KIND, int NB>
-__device__ __forceinline__ void accumulate_tile(const InterpArgs& a, int4 c0, int4 c1, uint4 av,
-                                                uint4 dv0, uint4 dv1, uint32_t w,
-                                                float4_t (&acc_re)[NB], float4_t (&acc_im)[NB],
-                                                int lane) {
-  const int n = lane & 15;
-  const int cols[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+// Buffer resources of one launch (wave-uniform, in scalar registers).  Every
+// per-tile operand is a buffer load at a scalar tile offset plus a
+// loop-invariant lane offset, and every gather one buffer load whose lane
+// offset is one VALU op from the stored column word, so the tile loop spends
+// almost no vector instructions on addresses.  Out-of-range offsets read 0.
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, uint64_t bytes) {
+  const uint32_t n = bytes > 0xFFFFFFFFull ? 0xFFFFFFFFu : static_cast<uint32_t>(bytes);
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), static_cast<short>(0), static_cast<int>(n),
+                                           0x00020000);
+}
+
+template <int NB>
+struct TileRsrc {
+  rsrc_t cols, a, idx, dense, rows, hi[NB], res[NB];
+};
+
+template <int NB, bool IL>
+__device__ __forceinline__ TileRsrc<NB> make_tile_rsrc(const InterpArgs& a) {
+  TileRsrc<NB> r;
+  const uint64_t tiles = a.n_tiles;
+  r.cols = make_rsrc(a.tile_cols, tiles * kK * 4);
+  r.a = make_rsrc(a.tile_a, tiles * 64 * 16);
+  r.idx = make_rsrc(a.tile_idx, tiles * 64 * 2);
+  r.dense = make_rsrc(a.tile_dense, a.tile_dense ? tiles * 64 * 32 : 0);
+  r.rows = make_rsrc(a.group_rows, static_cast<uint64_t>(a.n_groups) * 64);
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    const _Float16* ph = a.panel_hi + b * a.panel_stride + n;
-    half8_t bh;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) bh[j] = ph[static_cast<size_t>(cols[j]) * kN];
-    half8_t br;
-    if constexpr (KIND != kSparseHi) {
-      const _Float16* pr = a.panel_res + b * a.panel_stride + n;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) br[j] = pr[static_cast<size_t>(cols[j]) * kN];
-    }
-    if constexpr (KIND == kDenseSplit) {
-      const half8_t dre = __builtin_bit_cast(half8_t, dv0);
-      const half8_t dim = __builtin_bit_cast(half8_t, dv1);
-      acc_re[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(dre, bh, acc_re[b], 0, 0, 0);
-      acc_im[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(dim, bh, acc_im[b], 0, 0, 0);
-      acc_re[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(dre, br, acc_re[b], 0, 0, 0);
-      acc_im[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(dim, br, acc_im[b], 0, 0, 0);
+    if constexpr (IL) {
+      r.hi[b] = make_rsrc(a.panel_il + b * a.panel_stride, a.panel_stride * 4);
+      r.res[b] = r.hi[b];
     } else {
-      const half4_t are = __builtin_bit_cast(half4_t, make_uint2(av.x, av.y));
-      const half4_t aim = __builtin_bit_cast(half4_t, make_uint2(av.z, av.w));
-      const int idx = static_cast<int>(w);
-      acc_re[b] = __builtin_amdgcn_smfmac_f32_16x16x32_f16(are, bh, acc_re[b], idx, 0, 0);
-      acc_im[b] = __builtin_amdgcn_smfmac_f32_16x16x32_f16(aim, bh, acc_im[b], idx, 0, 1);
+      r.hi[b] = make_rsrc(a.panel_hi + b * a.panel_stride, a.panel_stride * 2);
+      r.res[b] = make_rsrc(a.panel_res ? a.panel_res + b * a.panel_stride : a.panel_hi, a.panel_stride * 2);
+    }
+  }
+  return r;
+}
+
+// Output rows of the lane's four accumulator rows (4q .. 4q+3) of group g.
+template <int NB>
+__device__ __forceinline__ int4 load_rows(const TileRsrc<NB>& r, int g, int lane) {
+  const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r.rows, (lane >> 4) * 16, g * 64, 0);
+  return make_int4(static_cast<int>(v.x), static_cast<int>(v.y), static_cast<int>(v.z), static_cast<int>(v.w));
+}
+
+__device__ __forceinline__ int4 group_rows_of(const InterpArgs& a, uint32_t group, int lane) {
+  return *reinterpret_cast<const int4*>(a.group_rows + static_cast<size_t>(group) * 16 + (lane >> 4) * 4);
+}
+
+// Stage 1 of a tile: the lane's 8 device column words (tile columns
+// 8q .. 8q+7).  A word holds the byte offset of the column's row in a half
+// panel (column * 32); bit 31 of word 2 is set on the last tile of a group.
+struct TileCols {
+  u32x4_t c0, c1;
+};
+
+template <int NB>
+__device__ __forceinline__ TileCols fetch_cols(const TileRsrc<NB>& r, int t, int lane) {
+  TileCols c;
+  const int off = (lane >> 4) * 32;
+  c.c0 = __builtin_amdgcn_raw_buffer_load_b128(r.cols, off, t * (kK * 4), 0);
+  c.c1 = __builtin_amdgcn_raw_buffer_load_b128(r.cols, off + 16, t * (kK * 4), 0);
+  return c;
+}
+
+// Stage 2 of a tile: A fragment, sparse index bytes, last-tile flag and the
+// raw gathered panel words (kept unpacked until the multiply so no wait is
+// forced before then).  IL: one dword (hi | res << 16) per row; otherwise one
+// zero-extended half per row for hi (w[0..7]) and for the residual (w[8..15]).
+template <int NB, bool IL>
+struct TileOps {
+  u32x4_t a0, a1;
+  uint32_t idx, last;
+  uint32_t w[NB][IL ? 8 : 16];
+};
+
+template <int KIND, int NB, bool IL>
+__device__ __forceinline__ void fetch_ops(const TileRsrc<NB>& r, int t, const TileCols& c, int lane,
+                                          TileOps<NB, IL>& o) {
+  const uint32_t n = lane & 15;
+  const uint32_t cw[8] = {c.c0.x, c.c0.y, c.c0.z, c.c0.w, c.c1.x, c.c1.y, c.c1.z, c.c1.w};
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if constexpr (IL) {
+        // row byte offset in the interleaved panel: column * 64 + n * 4
+        const uint32_t off = (cw[j] << 1) | (n << 2);
+        o.w[b][j] = __builtin_amdgcn_raw_buffer_load_b32(r.hi[b], static_cast<int>(off), 0, 0);
+      } else {
+        const uint32_t off = (j == 2 ? (cw[j] & 0x7FFFFFFFu) : cw[j]) | (n << 1);
+        o.w[b][j] = __builtin_amdgcn_raw_buffer_load_b16(r.hi[b], static_cast<int>(off), 0, 0);
+        o.w[b][8 + j] = KIND == kSparseHi ? 0u
+                                          : __builtin_amdgcn_raw_buffer_load_b16(r.res[b], static_cast<int>(off), 0, 0);
+      }
+    }
+  }
+  if constexpr (KIND == kDenseSplit) {
+    o.a0 = __builtin_amdgcn_raw_buffer_load_b128(r.dense, lane * 32, t * (64 * 32), 0);
+    o.a1 = __builtin_amdgcn_raw_buffer_load_b128(r.dense, lane * 32 + 16, t * (64 * 32), 0);
+    o.idx = 0;
+  } else {
+    o.a0 = __builtin_amdgcn_raw_buffer_load_b128(r.a, lane * 16, t * (64 * 16), 0);
+    o.a1 = u32x4_t{0u, 0u, 0u, 0u};
+    o.idx = __builtin_amdgcn_raw_buffer_load_b16(r.idx, lane * 2, t * (64 * 2), 0);
+  }
+  o.last = c.c0.z >> 31;
+}
+
+// hi (sel 0x05040100) or residual (sel 0x07060302) halves of 8 words as half8.
+__device__ __forceinline__ half8_t pack_halves(const uint32_t* w, uint32_t sel) {
+  u32x4_t v;
+  v.x = __builtin_amdgcn_perm(w[1], w[0], sel);
+  v.y = __builtin_amdgcn_perm(w[3], w[2], sel);
+  v.z = __builtin_amdgcn_perm(w[5], w[4], sel);
+  v.w = __builtin_amdgcn_perm(w[7], w[6], sel);
+  return __builtin_bit_cast(half8_t, v);
+}
+constexpr uint32_t kSelLo = 0x05040100u, kSelHi = 0x07060302u;
+
+template <int KIND, int NB, bool IL>
+__device__ __forceinline__ void mma_tile(const TileOps<NB, IL>& o, float4_t (&acc_re)[NB],
+                                         float4_t (&acc_im)[NB]) {
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const half8_t hi = pack_halves(o.w[b], kSelLo);
+    half8_t res;
+    if constexpr (IL) res = pack_halves(o.w[b], kSelHi);
+    else res = pack_halves(o.w[b] + 8, kSelLo);
+    if constexpr (KIND == kDenseSplit) {
+      const half8_t dre = __builtin_bit_cast(half8_t, o.a0);
+      const half8_t dim = __builtin_bit_cast(half8_t, o.a1);
+      acc_re[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(dre, hi, acc_re[b], 0, 0, 0);
+      acc_im[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(dim, hi, acc_im[b], 0, 0, 0);
+      acc_re[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(dre, res, acc_re[b], 0, 0, 0);
+      acc_im[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(dim, res, acc_im[b], 0, 0, 0);
+    } else {
+      const half4_t are = __builtin_bit_cast(half4_t, make_uint2(o.a0.x, o.a0.y));
+      const half4_t aim = __builtin_bit_cast(half4_t, make_uint2(o.a0.z, o.a0.w));
+      const int idx = static_cast<int>(o.idx);
+      acc_re[b] = __builtin_amdgcn_smfmac_f32_16x16x32_f16(are, hi, acc_re[b], idx, 0, 0);
+      acc_im[b] = __builtin_amdgcn_smfmac_f32_16x16x32_f16(aim, hi, acc_im[b], idx, 0, 1);
       if constexpr (KIND == kSparseSplit) {
-        acc_re[b] = __builtin_amdgcn_smfmac_f32_16x16x32_f16(are, br, acc_re[b], idx, 0, 0);
-        acc_im[b] = __builtin_amdgcn_smfmac_f32_16x16x32_f16(aim, br, acc_im[b], idx, 0, 1);
+        acc_re[b] = __builtin_amdgcn_smfmac_f32_16x16x32_f16(are, res, acc_re[b], idx, 0, 0);
+        acc_im[b] = __builtin_amdgcn_smfmac_f32_16x16x32_f16(aim, res, acc_im[b], idx, 0, 1);
       }
     }
   }
 }
 
-// Accumulate tiles first, first + stride, ... < end, fetching the next tile's
-// metadata while the current tile gathers.
-template <int KIND, int NB>
-__device__ __forceinline__ void run_tiles(const InterpArgs& a, int first, int end, int stride,
-                                          float4_t (&acc_re)[NB], float4_t (&acc_im)[NB], int lane) {
-  const int q = lane >> 4;
-  int4 c0 = {0, 0, 0, 0}, c1 = {0, 0, 0, 0};
-  uint4 av = {0, 0, 0, 0}, dv0 = {0, 0, 0, 0}, dv1 = {0, 0, 0, 0};
-  uint32_t w = 0;
-  auto fetch = [&](int t) {
-    const int4* cp = reinterpret_cast<const int4*>(a.tile_cols + static_cast<size_t>(t) * kK + q * 8);
-    c0 = cp[0];
-    c1 = cp[1];
-    if constexpr (KIND == kDenseSplit) {
-      dv0 = a.tile_dense[(static_cast<size_t>(t) * 64 + lane) * 2];
-      dv1 = a.tile_dense[(static_cast<size_t>(t) * 64 + lane) * 2 + 1];
-    } else {
-      av = a.tile_a[static_cast<size_t>(t) * 64 + lane];
-      w = a.tile_idx[static_cast<size_t>(t) * 64 + lane];
+// Gather stages in flight ahead of the multiply, as deep as 128 registers per
+// lane (the cap of a 16-wave block) hold without spilling: 2 for one coil
+// block of the interleaved panel, 0 for two coil blocks of separate panels or
+// of dense-control tiles, else 1.
+template <int KIND, int NB, bool IL>
+struct PipeDepth {
+  static constexpr int value = (IL && NB == 1 && KIND != kDenseSplit) ? 2
+                               : (NB == 2 && (!IL || KIND == kDenseSplit)) ? 0
+                                                                          : 1;
+};
+
+// Stream tiles first, first + stride, ... (< end) of one wave through a
+// software pipeline.  Step i issues the column words of tile i + D + 1, then
+// the operand loads of tile i + D, then multiplies tile i (D = PipeDepth);
+// loads return in issue order, so the wait for the words of tile i + D leaves
+// the operand loads of D tiles in flight.  Tile i lives in register slot
+// i % U of both rings (U = max(D + 1, 2), the loop unrolled U times so every
+// slot index is a constant).  Indices past the end are clamped to the last
+// tile (valid, unused loads).  GROUPS: the tiles are those of the consecutive
+// groups group, group + 1, ... (< group_end); the last tile of each runs the
+// epilogue with the group's rows (loaded one group ahead) and resets the
+// accumulators.  Otherwise everything accumulates.
+template <int KIND, int NB, bool IL, bool GROUPS, int OM, bool CONJ, bool ACC>
+__device__ __forceinline__ void stream_tiles(const InterpArgs& a, const TileRsrc<NB>& r, int first, int end,
+                                             int stride, int group, int group_end,
+                                             float4_t (&acc_re)[NB], float4_t (&acc_im)[NB], int lane) {
+  constexpr int D = PipeDepth<KIND, NB, IL>::value;
+  constexpr int U = D + 1 > 2 ? D + 1 : 2;
+  // wave-uniform bounds: scalar control flow and scalar tile offsets
+  first = __builtin_amdgcn_readfirstlane(first);
+  end = __builtin_amdgcn_readfirstlane(end);
+  int g = __builtin_amdgcn_readfirstlane(group);
+  const int g_last = __builtin_amdgcn_readfirstlane(group_end) - 1;
+  if (first >= end) return;
+  const int n = (end - first + stride - 1) / stride;
+  const int last = first + (n - 1) * stride;
+  auto tile = [&](int i) { return min(first + i * stride, last); };
+  int4 rows = make_int4(-1, -1, -1, -1);
+  if constexpr (GROUPS) rows = load_rows(r, g, lane);
+  TileCols cols[U];
+  TileOps<NB, IL> ops[U];
+  // The scheduler must keep the issue order: the wait counter is in order, and
+  // the loop's waits are computed over both the prologue and the back edge.
+#pragma unroll
+  for (int i = 0; i <= D; ++i) cols[i % U] = fetch_cols(r, tile(i), lane);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    fetch_ops<KIND, NB, IL>(r, tile(i), cols[i % U], lane, ops[i % U]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  for (int base = 0; base < n; base += U) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const int i = base + j;
+      __builtin_amdgcn_sched_barrier(0);
+      // loads run unconditionally (clamped), only the multiply is skipped past
+      // the end, so no loaded register is merged across a branch
+      cols[(j + D + 1) % U] = fetch_cols(r, tile(i + D + 1), lane);
+      __builtin_amdgcn_sched_barrier(0);
+      fetch_ops<KIND, NB, IL>(r, tile(i + D), cols[(j + D) % U], lane, ops[(j + D) % U]);
+      if (i < n) {
+        mma_tile<KIND, NB, IL>(ops[j], acc_re, acc_im);
+        if constexpr (GROUPS) {
+          if (__builtin_amdgcn_readfirstlane(ops[j].last)) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+              interp_epilogue<OM, CONJ, ACC>(a, rows, b, acc_re[b], acc_im[b], lane);
+              acc_re[b] = float4_t{0.f, 0.f, 0.f, 0.f};
+              acc_im[b] = float4_t{0.f, 0.f, 0.f, 0.f};
+            }
+            g = min(g + 1, g_last);
+            rows = load_rows(r, g, lane);
+          }
+        }
+      }
     }
-  };
-  if (first < end) fetch(first);
-  for (int t = first; t < end; t += stride) {
-    const int4 k0 = c0, k1 = c1;
-    const uint4 kav = av, kd0 = dv0, kd1 = dv1;
-    const uint32_t kw = w;
-    if (t + stride < end) fetch(t + stride);
-    accumulate_tile<KIND, NB>(a, k0, k1, kav, kd0, kd1, kw, acc_re, acc_im, lane);
   }
 }
 
 // Blocks of kTileWaves waves.  The first n_long blocks each own one long group
 // (waves stride over its tiles and are summed through LDS in wave order); the
-// other blocks hold one ordinary group per wave, XCD-contiguous.
-template <int KIND, int NB, int OM, bool CONJ, bool ACC>
+// other blocks hold one work unit (a run of whole groups) per wave, the units
+// of one XCD contiguous in pack order.
+template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC>
 __global__ __launch_bounds__(kTileWaves * 64) void interp_tiles_kernel(InterpArgs a) {
   __shared__ float4_t red[kTileWaves][NB][2][64];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   float4_t acc_re[NB], acc_im[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -180,7 +352,9 @@ __global__ __launch_bounds__(kTileWaves * 64) void interp_tiles_kernel(InterpArg
   if (blockIdx.x < a.n_long) {
     const WorkItem it = a.long_items[blockIdx.x];
     if (it.group < 0) return;  // padding block: uniform exit before the barrier
-    run_tiles<KIND, NB>(a, it.tile_begin + wave, it.tile_end, kTileWaves, acc_re, acc_im, lane);
+    const TileRsrc<NB> r = make_tile_rsrc<NB, IL>(a);
+    stream_tiles<KIND, NB, IL, false, OM, CONJ, ACC>(a, r, it.tile_begin + wave, it.tile_end, kTileWaves,
+                                                     it.group, it.group_end, acc_re, acc_im, lane);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       red[wave][b][0][lane] = acc_re[b];
@@ -188,6 +362,7 @@ __global__ __launch_bounds__(kTileWaves * 64) void interp_tiles_kernel(InterpArg
     }
     __syncthreads();
     if (wave != 0) return;
+    const int4 rows = group_rows_of(a, static_cast<uint32_t>(it.group), lane);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       float4_t re = red[0][b][0][lane], im = red[0][b][1][lane];
@@ -195,17 +370,23 @@ __global__ __launch_bounds__(kTileWaves * 64) void interp_tiles_kernel(InterpArg
         re += red[w2][b][0][lane];
         im += red[w2][b][1][lane];
       }
-      interp_epilogue<OM, CONJ, ACC>(a, it.group, b, re, im, lane);
+      interp_epilogue<OM, CONJ, ACC>(a, rows, b, re, im, lane);
     }
     return;
   }
   const uint32_t rb = blockIdx.x - a.n_long;
   const uint32_t item_id = xcd_remap(rb, gridDim.x - a.n_long) * kTileWaves + wave;
   if (item_id >= a.n_items) return;
-  const WorkItem it = a.items[item_id];
-  run_tiles<KIND, NB>(a, it.tile_begin, it.tile_end, 1, acc_re, acc_im, lane);
+  const WorkItem u = a.items[item_id];
+  if (u.tile_begin == u.tile_end) {  // empty group: zero rows
+    const int4 rows = group_rows_of(a, static_cast<uint32_t>(u.group), lane);
 #pragma unroll
-  for (int b = 0; b < NB; ++b) interp_epilogue<OM, CONJ, ACC>(a, it.group, b, acc_re[b], acc_im[b], lane);
+    for (int b = 0; b < NB; ++b) interp_epilogue<OM, CONJ, ACC>(a, rows, b, acc_re[b], acc_im[b], lane);
+    return;
+  }
+  const TileRsrc<NB> r = make_tile_rsrc<NB, IL>(a);
+  stream_tiles<KIND, NB, IL, true, OM, CONJ, ACC>(a, r, u.tile_begin, u.tile_end, 1, u.group, u.group_end,
+                                                  acc_re, acc_im, lane);
 }
 
 struct CsrArgs {
@@ -284,13 +465,33 @@ __global__ __launch_bounds__(256) void interp_csr_kernel(CsrArgs a) {
   if (idx >= a.n_short) return;
   const int32_t task = a.short_rows[idx];
   const uint32_t begin = a.row_ptr[task], end = a.row_ptr[task + 1];
+  const uint32_t cc = active ? c : 0;  // inactive lanes read coil 0 and discard it
   float re0 = 0.f, im0 = 0.f, re1 = 0.f, im1 = 0.f;
   uint32_t k = begin;
-  for (; k + 2 <= end; k += 2) {
-    csr_accumulate<CONJ>(a, k, c, active, re0, im0);
-    csr_accumulate<CONJ>(a, k + 1, c, active, re1, im1);
+  for (; k + 4 <= end; k += 4) {
+    int32_t col[4];
+    float2 v[4], x[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      col[i] = a.cols[k + i];
+      v[i] = a.vals[k + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) x[i] = a.in[static_cast<size_t>(col[i]) * a.coils + cc];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      float& re = (i & 1) ? re1 : re0;
+      float& im = (i & 1) ? im1 : im0;
+      if (CONJ) {
+        re += v[i].x * x[i].x + v[i].y * x[i].y;
+        im += v[i].x * x[i].y - v[i].y * x[i].x;
+      } else {
+        re += v[i].x * x[i].x - v[i].y * x[i].y;
+        im += v[i].x * x[i].y + v[i].y * x[i].x;
+      }
+    }
   }
-  if (k < end) csr_accumulate<CONJ>(a, k, c, active, re0, im0);
+  for (; k < end; ++k) csr_accumulate<CONJ>(a, k, cc, true, re0, im0);
   if (!active) return;
   csr_store<OM, ACC>(a, a.row_ids[task], c, re0 + re1, im0 + im1);
 }
@@ -319,49 +520,90 @@ __global__ __launch_bounds__(256) void split_panel_kernel(const float2* __restri
   res[gtid] = static_cast<_Float16>(v - static_cast<float>(h));
 }
 
-// ------------------------------------------------------------- dispatch
-
-template <int KIND, int NB, int OM>
-static void launch_tiles_om(const InterpArgs& a, bool conj, bool acc, hipStream_t s) {
-  const uint32_t blocks = a.n_long + div_up(a.n_items, kTileWaves);
-#define TSN_LAUNCH(C, A)                                                                     \
-  do {                                                                                       \
-    if (blocks) hipLaunchKernelGGL((interp_tiles_kernel<KIND, NB, OM, C, A>), dim3(blocks),  \
-                                   dim3(kTileWaves * 64), 0, s, a);                          \
-  } while (0)
-  if (conj) {
-    if (acc) TSN_LAUNCH(true, true); else TSN_LAUNCH(true, false);
-  } else {
-    if (acc) TSN_LAUNCH(false, true); else TSN_LAUNCH(false, false);
+// Interleaved variant for the plan: one uint32 = hi | res << 16 per channel.
+__global__ __launch_bounds__(256) void split_panel_il_kernel(const float2* __restrict__ in, uint32_t rows,
+                                                             uint32_t coils, uint32_t blocks, float scale,
+                                                             uint32_t* __restrict__ out) {
+  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t total = static_cast<uint64_t>(blocks) * rows * kN;
+  if (gtid >= total) return;
+  const int n = static_cast<int>(gtid & 15);
+  const uint64_t br = gtid >> 4;
+  const uint32_t row = static_cast<uint32_t>(br % rows);
+  const uint32_t b = static_cast<uint32_t>(br / rows);
+  const uint32_t coil = b * kCoilsPerBlock + (n & 7);
+  float v = 0.f;
+  if (coil < coils) {
+    const float2 x = in[static_cast<size_t>(coil) * rows + row];
+    v = (n < 8 ? x.x : x.y) * scale;
   }
-#undef TSN_LAUNCH
+  const _Float16 h = static_cast<_Float16>(v);
+  const _Float16 r = static_cast<_Float16>(v - static_cast<float>(h));
+  out[gtid] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
+              (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r)) << 16);
 }
 
-template <int KIND, int NB>
+// ------------------------------------------------------------- dispatch
+
+template <int KIND, int NB, bool IL, int OM>
+static void launch_tiles_om(const InterpArgs& a, bool conj, bool acc, hipStream_t s) {
+  const uint32_t blocks = a.n_long + div_up(a.n_items, kTileWaves);
+  if (!blocks) return;
+  const dim3 grid(blocks), block(kTileWaves * 64);
+  constexpr bool kAccOk = OM != kOutSplit16 && OM != kOutSplit16IL;
+  if (conj) {
+    if (kAccOk && acc) hipLaunchKernelGGL((interp_tiles_kernel<KIND, NB, IL, OM, true, kAccOk>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((interp_tiles_kernel<KIND, NB, IL, OM, true, false>), grid, block, 0, s, a);
+  } else {
+    if (kAccOk && acc) hipLaunchKernelGGL((interp_tiles_kernel<KIND, NB, IL, OM, false, kAccOk>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((interp_tiles_kernel<KIND, NB, IL, OM, false, false>), grid, block, 0, s, a);
+  }
+}
+
+// Interleaved panels (plan-internal) pair with FP32 or interleaved split
+// output; separate panels with FP32 or separate split output (checked by
+// launch_interp).
+template <int KIND, int NB, bool IL>
 static void launch_tiles_nb(const InterpArgs& a, int om, bool conj, bool acc, hipStream_t s) {
-  if (om == kOutCoilMajor) launch_tiles_om<KIND, NB, kOutCoilMajor>(a, conj, acc, s);
-  else if (om == kOutRowMajor) launch_tiles_om<KIND, NB, kOutRowMajor>(a, conj, acc, s);
-  else launch_tiles_om<KIND, NB, kOutSplit16>(a, conj, false, s);
+  if constexpr (IL) {
+    if (om == kOutCoilMajor) launch_tiles_om<KIND, NB, IL, kOutCoilMajor>(a, conj, acc, s);
+    else if (om == kOutRowMajor) launch_tiles_om<KIND, NB, IL, kOutRowMajor>(a, conj, acc, s);
+    else launch_tiles_om<KIND, NB, IL, kOutSplit16IL>(a, conj, false, s);
+  } else {
+    if (om == kOutCoilMajor) launch_tiles_om<KIND, NB, IL, kOutCoilMajor>(a, conj, acc, s);
+    else if (om == kOutRowMajor) launch_tiles_om<KIND, NB, IL, kOutRowMajor>(a, conj, acc, s);
+    else launch_tiles_om<KIND, NB, IL, kOutSplit16>(a, conj, false, s);
+  }
 }
 
 template <int KIND>
-static void launch_tiles(const InterpArgs& a, int om, bool conj, bool acc, hipStream_t s) {
-  if (a.coils > kCoilsPerBlock) launch_tiles_nb<KIND, 2>(a, om, conj, acc, s);
-  else launch_tiles_nb<KIND, 1>(a, om, conj, acc, s);
+static void launch_tiles(const InterpArgs& a, bool il, int om, bool conj, bool acc, hipStream_t s) {
+  const bool two = a.coils > kCoilsPerBlock;
+  if (il) {
+    if (two) launch_tiles_nb<KIND, 2, true>(a, om, conj, acc, s);
+    else launch_tiles_nb<KIND, 1, true>(a, om, conj, acc, s);
+  } else {
+    if (two) launch_tiles_nb<KIND, 2, false>(a, om, conj, acc, s);
+    else launch_tiles_nb<KIND, 1, false>(a, om, conj, acc, s);
+  }
 }
 
 // Internal entry used by the plan and the C ABI.  All pointers are device
 // pointers sized by the caller; panels hold ceil(coils/8) blocks.
 int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* res, uint32_t coils,
                   int out_mode, bool conj, bool accumulate, float scale, float* out_f32,
-                  _Float16* out_hi, _Float16* out_res, hipStream_t stream) {
+                  _Float16* out_hi, _Float16* out_res, hipStream_t stream, bool interleaved,
+                  uint32_t* out_il) {
   if (!d || d->csr) return fail(TSN_ERR_ARGUMENT, "interpolation needs a packed (non-CSR) direction");
   if (coils == 0 || coils > static_cast<uint32_t>(kMaxCoils))
     return fail(TSN_ERR_ARGUMENT, "coils must be in 1..16 (got %u)", coils);
-  if (!hi || (kind != kSparseHi && !res))
+  if (!hi || (!interleaved && kind != kSparseHi && !res))
     return fail(TSN_ERR_ARGUMENT, "missing input panel");
-  if (out_mode == kOutSplit16 ? (!out_hi || !out_res) : !out_f32)
+  const bool split_out = out_mode == kOutSplit16 || out_mode == kOutSplit16IL;
+  if (split_out ? (interleaved ? !out_il : (!out_hi || !out_res)) : !out_f32)
     return fail(TSN_ERR_ARGUMENT, "missing output");
+  if (interleaved ? out_mode == kOutSplit16 : out_mode == kOutSplit16IL)
+    return fail(TSN_ERR_ARGUMENT, "output mode %d does not match the panel layout", out_mode);
   if (kind == kDenseSplit && !d->tile_dense)
     return fail(TSN_ERR_ARGUMENT, "direction has no dense-control tiles");
   InterpArgs a{};
@@ -370,12 +612,16 @@ int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* 
   a.long_items = d->long_items;
   a.n_long = d->n_long;
   a.group_rows = d->group_rows;
+  a.group_offsets = d->group_offsets;
   a.tile_cols = d->tile_cols;
+  a.n_tiles = d->h.tiles;
+  a.n_groups = d->h.groups;
   a.tile_a = d->tile_a;
   a.tile_idx = d->tile_idx;
   a.tile_dense = d->tile_dense;
   a.panel_hi = static_cast<const _Float16*>(hi);
   a.panel_res = static_cast<const _Float16*>(res);
+  a.panel_il = static_cast<const uint32_t*>(hi);
   a.panel_stride = static_cast<uint64_t>(d->h.n_cols) * kN;
   a.coils = coils;
   a.n_out = d->h.output_size;
@@ -383,9 +629,10 @@ int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* 
   a.out_f32 = out_f32;
   a.out_hi = out_hi;
   a.out_res = out_res;
-  if (kind == kSparseSplit) launch_tiles<kSparseSplit>(a, out_mode, conj, accumulate, stream);
-  else if (kind == kDenseSplit) launch_tiles<kDenseSplit>(a, out_mode, conj, accumulate, stream);
-  else launch_tiles<kSparseHi>(a, out_mode, conj, accumulate, stream);
+  a.out_il = out_il;
+  if (kind == kSparseSplit) launch_tiles<kSparseSplit>(a, interleaved, out_mode, conj, accumulate, stream);
+  else if (kind == kDenseSplit) launch_tiles<kDenseSplit>(a, interleaved, out_mode, conj, accumulate, stream);
+  else launch_tiles<kSparseHi>(a, interleaved, out_mode, conj, accumulate, stream);
   TSN_HIP(hipGetLastError());
   return TSN_OK;
 }
@@ -441,6 +688,19 @@ int launch_split_panel(const float2* in, uint32_t rows, uint32_t coils, float sc
   return TSN_OK;
 }
 
+int launch_split_panel_il(const float2* in, uint32_t rows, uint32_t coils, float scale, uint32_t* out,
+                          hipStream_t stream) {
+  if (!in || !out || rows == 0) return fail(TSN_ERR_ARGUMENT, "bad split arguments");
+  if (coils == 0 || coils > static_cast<uint32_t>(kMaxCoils))
+    return fail(TSN_ERR_ARGUMENT, "coils must be in 1..16 (got %u)", coils);
+  const uint32_t blocks = div_up(coils, kCoilsPerBlock);
+  const uint64_t total = static_cast<uint64_t>(blocks) * rows * kN;
+  hipLaunchKernelGGL(split_panel_il_kernel, dim3(div_up(total, 256)), dim3(256), 0, stream, in, rows,
+                     coils, blocks, scale, out);
+  TSN_HIP(hipGetLastError());
+  return TSN_OK;
+}
+
 // ------------------------------------------------------------- upload
 
 template <typename T>
@@ -460,7 +720,85 @@ static void release(tsn_direction* d) {
                   d->cols, d->vals, d->short_rows, d->long_rows};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (void* p : d->retired) (void)hipFree(p);
   delete d;
+}
+
+// Work schedule of a packed direction.  Groups longer than kSplitAboveTiles
+// tiles are units owned by a whole block (padded with group -1 entries to a
+// multiple of 8 blocks so the other blocks keep their XCD mapping).  The other
+// groups are cut, in pack order, into contiguous runs of whole groups holding
+// at most unit_tiles tiles (a single group is never cut); an empty group is a
+// unit of its own (zero epilogue).  unit_tiles 0 picks the default: the
+// regular tiles spread over 1.5 rounds of resident waves (one 16-wave block
+// per CU), clamped to 2..8 tiles.  Measured at 205,824 samples / 512^2 grid:
+// 8-tile units beat both one full round (12-tile units leave a tiny second
+// round) and short units (more per-unit start-up), 26 us against 35 us.
+static int schedule_units(tsn_direction* d, uint32_t unit_tiles) {
+  const std::vector<uint32_t>& off = d->host_offsets;
+  const uint32_t groups = d->h.groups;
+  uint64_t regular_tiles = 0;
+  for (uint32_t g = 0; g < groups; ++g) {
+    const uint32_t n = off[g + 1] - off[g];
+    if (n <= static_cast<uint32_t>(kSplitAboveTiles)) regular_tiles += n;
+  }
+  int64_t target = unit_tiles;
+  if (target == 0) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device) != hipSuccess || cus <= 0)
+      cus = 256;
+    const uint64_t waves = static_cast<uint64_t>(cus) * kTileWaves * 3 / 2;
+    target = std::clamp<int64_t>(static_cast<int64_t>((regular_tiles + waves - 1) / waves), 2, 8);
+  }
+  std::vector<WorkItem> items, long_items;
+  WorkItem cur{-1, -1, 0, 0};
+  auto flush = [&]() {
+    if (cur.group >= 0) items.push_back(cur);
+    cur = {-1, -1, 0, 0};
+  };
+  for (uint32_t g = 0; g < groups; ++g) {
+    const int32_t b = static_cast<int32_t>(off[g]);
+    const int32_t e = static_cast<int32_t>(off[g + 1]);
+    const int32_t gi = static_cast<int32_t>(g);
+    if (e - b > kSplitAboveTiles) {
+      flush();
+      long_items.push_back({gi, gi + 1, b, e});
+      continue;
+    }
+    if (e == b) {
+      flush();
+      items.push_back({gi, gi + 1, b, e});
+      continue;
+    }
+    if (cur.group >= 0 && cur.tile_end - cur.tile_begin + (e - b) > target) flush();
+    if (cur.group < 0) {
+      cur = {gi, gi + 1, b, e};
+    } else {
+      cur.group_end = gi + 1;
+      cur.tile_end = e;
+    }
+  }
+  flush();
+  while (long_items.size() % 8) long_items.push_back({-1, -1, 0, 0});
+  WorkItem *new_items = nullptr, *new_long = nullptr;
+  size_t bytes = 0;
+  int rc = device_copy(items.data(), items.size(), &new_items, &bytes);
+  if (rc == TSN_OK) rc = device_copy(long_items.data(), long_items.size(), &new_long, &bytes);
+  if (rc != TSN_OK) {
+    if (new_items) (void)hipFree(new_items);
+    return rc;
+  }
+  // Earlier schedules stay allocated until the direction is freed: a captured
+  // graph may still hold their addresses in its kernel arguments.
+  if (d->items) d->retired.push_back(d->items);
+  if (d->long_items) d->retired.push_back(d->long_items);
+  d->items = new_items;
+  d->long_items = new_long;
+  d->n_items = static_cast<uint32_t>(items.size());
+  d->n_long = static_cast<uint32_t>(long_items.size());
+  d->unit_tiles = static_cast<uint32_t>(target);
+  d->bytes += bytes;
+  return TSN_OK;
 }
 
 static int upload_pack(const HostPack& p, tsn_direction** out) {
@@ -506,35 +844,36 @@ static int upload_pack(const HostPack& p, tsn_direction** out) {
       tile_dense[(t * 64 + lane) * 2 + 1] = d1;
     }
   }
-  // ordinary groups: one wave each; long groups: one block each (padded to a
-  // multiple of 8 blocks so the ordinary blocks keep their XCD mapping)
-  std::vector<WorkItem> items, long_items;
+  // Device column words (see TileCols): byte offset of the column's row in a
+  // half panel, bit 31 of word 2 of each lane segment = last tile of a group.
+  if (h.n_cols > kMaxDeviceCols)
+    return fail(TSN_ERR_ARGUMENT, "pack has %u columns, the device format holds at most %u", h.n_cols,
+                kMaxDeviceCols);
+  std::vector<uint32_t> dev_cols(p.tile_col_ids.size());
+  for (size_t i = 0; i < dev_cols.size(); ++i) dev_cols[i] = static_cast<uint32_t>(p.tile_col_ids[i]) * 32u;
   for (uint32_t g = 0; g < h.groups; ++g) {
-    const int32_t b = static_cast<int32_t>(p.group_offsets[g]);
-    const int32_t e = static_cast<int32_t>(p.group_offsets[g + 1]);
-    (e - b <= kSplitAboveTiles ? items : long_items).push_back({static_cast<int32_t>(g), b, e, 0});
+    if (p.group_offsets[g + 1] == p.group_offsets[g]) continue;
+    const size_t t = p.group_offsets[g + 1] - 1;
+    for (int q = 0; q < 4; ++q) dev_cols[t * kK + q * 8 + 2] |= 0x80000000u;
   }
-  while (long_items.size() % 8) long_items.push_back({-1, 0, 0, 0});
   auto* d = new tsn_direction();
   d->h = h;
   (void)hipGetDevice(&d->device);
-  d->n_items = static_cast<uint32_t>(items.size());
-  d->n_long = static_cast<uint32_t>(long_items.size());
+  d->host_offsets = p.group_offsets;
   size_t bytes = 0;
   int rc = TSN_OK;
   if (rc == TSN_OK) rc = device_copy(p.group_offsets.data(), p.group_offsets.size(), &d->group_offsets, &bytes);
   if (rc == TSN_OK) rc = device_copy(p.group_row_ids.data(), p.group_row_ids.size(), &d->group_rows, &bytes);
-  if (rc == TSN_OK) rc = device_copy(p.tile_col_ids.data(), p.tile_col_ids.size(), &d->tile_cols, &bytes);
+  if (rc == TSN_OK) rc = device_copy(dev_cols.data(), dev_cols.size(), &d->tile_cols, &bytes);
   if (rc == TSN_OK) rc = device_copy(tile_a.data(), tile_a.size(), &d->tile_a, &bytes);
   if (rc == TSN_OK) rc = device_copy(tile_idx.data(), tile_idx.size(), &d->tile_idx, &bytes);
   if (rc == TSN_OK) rc = device_copy(tile_dense.data(), tile_dense.size(), &d->tile_dense, &bytes);
-  if (rc == TSN_OK) rc = device_copy(items.data(), items.size(), &d->items, &bytes);
-  if (rc == TSN_OK) rc = device_copy(long_items.data(), long_items.size(), &d->long_items, &bytes);
+  d->bytes = bytes;
+  if (rc == TSN_OK) rc = schedule_units(d, 0);
   if (rc != TSN_OK) {
     release(d);
     return rc;
   }
-  d->bytes = bytes;
   *out = d;
   return TSN_OK;
 }
@@ -548,6 +887,17 @@ int tsn_direction_upload(const tsn_host_pack* pack, tsn_direction** out) {
   *out = nullptr;
   return upload_pack(*pack, out);
 }
+
+int tsn_direction_schedule(tsn_direction* d, uint32_t unit_tiles) {
+  if (!d) return fail(TSN_ERR_ARGUMENT, "null direction");
+  if (d->csr) return fail(TSN_ERR_ARGUMENT, "a CSR direction has no tile schedule");
+  if (unit_tiles > (1u << 20)) return fail(TSN_ERR_ARGUMENT, "unit_tiles %u is out of range", unit_tiles);
+  TSN_HIP(hipSetDevice(d->device));
+  TSN_HIP(hipDeviceSynchronize());
+  return schedule_units(d, unit_tiles);
+}
+
+uint32_t tsn_direction_unit_tiles(const tsn_direction* d) { return d ? d->unit_tiles : 0; }
 
 int tsn_direction_load(const char* real_dir, const char* imag_dir, tsn_direction** out) {
   if (!out) return fail(TSN_ERR_ARGUMENT, "null argument");
@@ -657,7 +1007,7 @@ int tsn_interp(const tsn_direction* d, const void* hi, const void* res, void* ou
     return fail(TSN_ERR_ARGUMENT, "unknown output layout %d", out_layout);
   return launch_interp(d, res ? kSparseSplit : kSparseHi, hi, res, coils, out_layout, conjugate != 0,
                        accumulate != 0, scale, static_cast<float*>(out), nullptr, nullptr,
-                       static_cast<hipStream_t>(stream));
+                       static_cast<hipStream_t>(stream), false, nullptr);
 }
 
 int tsn_interp_dense(const tsn_direction* d, const void* hi, const void* res, void* out,
@@ -668,7 +1018,7 @@ int tsn_interp_dense(const tsn_direction* d, const void* hi, const void* res, vo
   if (!res) return fail(TSN_ERR_ARGUMENT, "dense control needs hi and residual panels");
   return launch_interp(d, kDenseSplit, hi, res, coils, out_layout, conjugate != 0, accumulate != 0,
                        scale, static_cast<float*>(out), nullptr, nullptr,
-                       static_cast<hipStream_t>(stream));
+                       static_cast<hipStream_t>(stream), false, nullptr);
 }
 
 int tsn_interp_f32(const tsn_direction* d, const void* in, void* out, uint32_t coils, int conjugate,
@@ -676,6 +1026,21 @@ int tsn_interp_f32(const tsn_direction* d, const void* in, void* out, uint32_t c
   return launch_interp_csr(d, static_cast<const float2*>(in), coils, out_layout, conjugate != 0,
                            accumulate != 0, scale, static_cast<float2*>(out),
                            static_cast<hipStream_t>(stream));
+}
+
+int tsn_interp_il(const tsn_direction* d, const void* panel, void* out, uint32_t coils, int conjugate,
+                  int accumulate, float scale, int out_layout, void* stream) {
+  if (out_layout != TSN_OUT_COIL_MAJOR && out_layout != TSN_OUT_ROW_MAJOR)
+    return fail(TSN_ERR_ARGUMENT, "unknown output layout %d", out_layout);
+  return launch_interp(d, kSparseSplit, panel, nullptr, coils, out_layout, conjugate != 0, accumulate != 0,
+                       scale, static_cast<float*>(out), nullptr, nullptr, static_cast<hipStream_t>(stream),
+                       true, nullptr);
+}
+
+int tsn_split_panel_il(const void* in, uint32_t rows, uint32_t coils, float scale, void* panel,
+                       void* stream) {
+  return launch_split_panel_il(static_cast<const float2*>(in), rows, coils, scale,
+                               static_cast<uint32_t*>(panel), static_cast<hipStream_t>(stream));
 }
 
 int tsn_split_panel(const void* in, uint32_t rows, uint32_t coils, float scale, void* hi, void* res,

@@ -21,7 +21,10 @@
 namespace tsn {
 int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* res, uint32_t coils,
                   int out_mode, bool conj, bool accumulate, float scale, float* out_f32,
-                  _Float16* out_hi, _Float16* out_res, hipStream_t stream);
+                  _Float16* out_hi, _Float16* out_res, hipStream_t stream, bool interleaved,
+                  uint32_t* out_il);
+int launch_split_panel_il(const float2* in, uint32_t rows, uint32_t coils, float scale, uint32_t* out,
+                          hipStream_t stream);
 int launch_interp_csr(const tsn_direction* d, const float2* in, uint32_t coils, int out_mode,
                       bool conj, bool accumulate, float scale, float2* out, hipStream_t stream);
 int launch_split_panel(const float2* in, uint32_t rows, uint32_t coils, float scale, _Float16* hi,
@@ -225,7 +228,7 @@ struct tsn_plan {
   std::vector<void*> allocations;
   float2 *tw0 = nullptr, *tw1 = nullptr, *smaps_t = nullptr, *t_fwd = nullptr, *t_inv = nullptr;
   float* apod_t = nullptr;
-  _Float16 *pan_hi = nullptr, *pan_res = nullptr, *smp_hi = nullptr, *smp_res = nullptr;
+  uint32_t *pan_il = nullptr, *smp_il = nullptr;  // interleaved hi | res << 16 panels
   float2 *pan32 = nullptr, *smp32 = nullptr, *grid = nullptr;
   float2 *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *b = nullptr, *tmp = nullptr;
   double* partial = nullptr;
@@ -269,8 +272,7 @@ FftArgs fft_args(const tsn_plan* pl) {
   a.smaps_t = pl->smaps_t;
   a.apod_t = pl->apod_t;
   a.t_fwd = pl->t_fwd;
-  a.panel_hi = pl->pan_hi;
-  a.panel_res = pl->pan_res;
+  a.panel_il = pl->pan_il;
   a.panel32 = pl->pan32;
   a.grid = pl->grid;
   a.t_inv = pl->t_inv;
@@ -307,18 +309,18 @@ int op_forward_samples(tsn_plan* pl, float2* out_coil_major, hipStream_t s) {
     return launch_interp_csr(pl->fwd, pl->pan32, pl->cfg.coils, 1, false, false, scale, pl->smp32, s);
   }
   if (out_coil_major)
-    return launch_interp(pl->fwd, interp_kind(pl), pl->pan_hi, pl->pan_res, pl->cfg.coils, 0, false,
-                         false, scale, reinterpret_cast<float*>(out_coil_major), nullptr, nullptr, s);
-  return launch_interp(pl->fwd, interp_kind(pl), pl->pan_hi, pl->pan_res, pl->cfg.coils, 2, false, false,
-                       scale, nullptr, pl->smp_hi, pl->smp_res, s);
+    return launch_interp(pl->fwd, interp_kind(pl), pl->pan_il, nullptr, pl->cfg.coils, 0, false, false, scale,
+                         reinterpret_cast<float*>(out_coil_major), nullptr, nullptr, s, true, nullptr);
+  return launch_interp(pl->fwd, interp_kind(pl), pl->pan_il, nullptr, pl->cfg.coils, 3, false, false, scale,
+                       nullptr, nullptr, nullptr, s, true, pl->smp_il);
 }
 
 // sample panels -> grid (conjugate transpose interpolation)
 int op_adjoint_grid(tsn_plan* pl, hipStream_t s) {
   if (is_fp32(pl))
     return launch_interp_csr(pl->adj, pl->smp32, pl->cfg.coils, 1, true, false, 1.0f, pl->grid, s);
-  return launch_interp(pl->adj, interp_kind(pl), pl->smp_hi, pl->smp_res, pl->cfg.coils, 1, true, false,
-                       1.0f, reinterpret_cast<float*>(pl->grid), nullptr, nullptr, s);
+  return launch_interp(pl->adj, interp_kind(pl), pl->smp_il, nullptr, pl->cfg.coils, 1, true, false, 1.0f,
+                       reinterpret_cast<float*>(pl->grid), nullptr, nullptr, s, true, nullptr);
 }
 
 // grid -> image (column-major) with combine, scale, regulariser, dot partial
@@ -350,7 +352,7 @@ int op_load_samples(tsn_plan* pl, const float2* samples, float scale, hipStream_
     TSN_HIP(hipGetLastError());
     return TSN_OK;
   }
-  return launch_split_panel(samples, pl->m, pl->cfg.coils, scale, pl->smp_hi, pl->smp_res, s);
+  return launch_split_panel_il(samples, pl->m, pl->cfg.coils, scale, pl->smp_il, s);
 }
 
 int transpose_image(const float2* in, uint32_t rows, uint32_t cols, float scale, float2* out, hipStream_t s) {
@@ -468,10 +470,8 @@ int tsn_plan_create(const tsn_plan_config* cfg, const tsn_direction* fwd, const 
     A(&pl->pan32, static_cast<size_t>(grid) * c.coils);
     A(&pl->smp32, static_cast<size_t>(pl->m) * c.coils);
   } else {
-    A(&pl->pan_hi, static_cast<size_t>(pl->blocks) * grid * kN);
-    A(&pl->pan_res, static_cast<size_t>(pl->blocks) * grid * kN);
-    A(&pl->smp_hi, static_cast<size_t>(pl->blocks) * pl->m * kN);
-    A(&pl->smp_res, static_cast<size_t>(pl->blocks) * pl->m * kN);
+    A(&pl->pan_il, static_cast<size_t>(pl->blocks) * grid * kN);
+    A(&pl->smp_il, static_cast<size_t>(pl->blocks) * pl->m * kN);
   }
   for (float2** v : {&pl->x, &pl->r, &pl->p, &pl->ap, &pl->b, &pl->tmp}) A(v, px);
   A(&pl->partial, std::max<size_t>(pl->vec_blocks, c.image1));

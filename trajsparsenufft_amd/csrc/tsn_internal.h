@@ -23,6 +23,9 @@ constexpr int kMaxCoils = 16;
 constexpr int kTileWaves = 16;        // waves per interpolation block (1024 threads)
 constexpr int kSplitAboveTiles = 16;  // longer groups get a whole block (waves stride over tiles)
 constexpr int kLongRowEntries = 128;  // FP32 CSR rows longer than this get a whole block
+// Device column words of packed tiles hold column * 32 (a byte offset) below
+// bit 31, so a pack may have at most 2^26 columns.
+constexpr uint32_t kMaxDeviceCols = 1u << 26;
 
 void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -68,16 +71,18 @@ inline void decode_row_meta(const uint32_t* pair_meta, uint32_t* row_meta) {
   }
 }
 
-// Work item of the interpolation kernels: the tile range of one group.
-// Groups longer than kSplitAboveTiles are listed separately and get a whole
-// block whose waves stride over the tiles; group < 0 marks a padding entry.
+// Work unit of the interpolation kernels: whole groups [group, group_end)
+// whose tiles [tile_begin, tile_end) one wave streams through a continuous
+// pipeline (epilogue at every group boundary).  Units are sized to a target
+// tile count so one round of waves covers the launch.  Groups longer than
+// kSplitAboveTiles are separate units owned by a whole block; group < 0 marks
+// a padding unit.
 struct WorkItem {
   int32_t group;
+  int32_t group_end;
   int32_t tile_begin;
   int32_t tile_end;
-  int32_t pad;
 };
-
 }  // namespace tsn
 
 struct tsn_host_pack : tsn::HostPack {};
@@ -97,13 +102,16 @@ struct tsn_direction {
   // ---- packed 2:4 tiles (device)
   uint32_t* group_offsets = nullptr;
   int32_t* group_rows = nullptr;     // [groups * 16] logical output rows, -1 = padding
-  int32_t* tile_cols = nullptr;      // [tiles * 32] panel rows
+  uint32_t* tile_cols = nullptr;     // [tiles * 32] column * 32 | last-tile flag (bit 31, word 2 per 8)
   uint4* tile_a = nullptr;           // [tiles * 64] per lane {re 4xf16, im 4xf16}
   uint16_t* tile_idx = nullptr;      // [tiles * 64] per lane sparse index byte re | im << 8
   uint4* tile_dense = nullptr;       // [tiles * 64 * 2] per lane {re 8xf16}, {im 8xf16}
-  tsn::WorkItem* items = nullptr;    // groups with <= kSplitAboveTiles tiles
+  tsn::WorkItem* items = nullptr;    // runs of groups with <= kSplitAboveTiles tiles each
   tsn::WorkItem* long_items = nullptr;  // longer groups, padded to a multiple of 8
   uint32_t n_items = 0, n_long = 0;
+  uint32_t unit_tiles = 0;           // tile budget of one regular unit
+  std::vector<uint32_t> host_offsets;  // group_offsets kept for rescheduling
+  std::vector<void*> retired;          // earlier schedules (graphs may reference them)
   // ---- FP32 CSR (device)
   uint32_t csr_rows = 0;
   uint32_t* row_ptr = nullptr;   // [csr_rows + 1]

@@ -79,6 +79,16 @@ class Direction:
     def device_bytes(self) -> int:
         return int(lib().tsn_direction_bytes(self._h))
 
+    @property
+    def unit_tiles(self) -> int:
+        """Tile budget of one regular work unit (0 for CSR directions)."""
+        return int(lib().tsn_direction_unit_tiles(self._h))
+
+    def schedule(self, unit_tiles: int = 0) -> None:
+        """Re-cut the work units (runs of whole groups of at most unit_tiles
+        tiles; 0 = default).  Results are bitwise independent of the cut."""
+        check(lib().tsn_direction_schedule(self._h, int(unit_tiles)))
+
     @classmethod
     def upload(cls, pack: HostPack) -> "Direction":
         out = C.c_void_p()
@@ -140,6 +150,28 @@ def split_panel(samples: torch.Tensor, coils: int, scale: float = 1.0):
     res = torch.empty_like(hi)
     check(lib().tsn_split_panel(_ptr(samples), rows, coils, float(scale), _ptr(hi), _ptr(res), _stream()))
     return hi, res
+
+
+def split_panel_il(samples: torch.Tensor, coils: int, scale: float = 1.0) -> torch.Tensor:
+    """complex64 [coils, rows] -> interleaved int32 [blocks, rows, 16] (hi | res << 16)."""
+
+    rows = samples.numel() // coils
+    _require(samples, torch.complex64, coils * rows, "samples")
+    panel = torch.empty((coil_blocks(coils), rows, 16), dtype=torch.int32, device=samples.device)
+    check(lib().tsn_split_panel_il(_ptr(samples), rows, coils, float(scale), _ptr(panel), _stream()))
+    return panel
+
+
+def interp_il(direction: Direction, panel: torch.Tensor, out: torch.Tensor, coils: int = 8,
+              conjugate: bool = False, accumulate: bool = False, scale: float = 1.0,
+              layout: str = "coil_major") -> torch.Tensor:
+    """Packed interpolation of an interleaved split panel (int32 [blocks, n_cols, 16])."""
+
+    _require(panel, torch.int32, coil_blocks(coils) * direction.cols * 16, "panel")
+    _require(out, torch.complex64, coils * direction.rows, "out")
+    check(lib().tsn_interp_il(direction.handle, _ptr(panel), _ptr(out), coils, int(conjugate), int(accumulate),
+                              float(scale), LAYOUTS[layout], _stream()))
+    return out
 
 
 # ------------------------------------------------------ reference FFT ABI
