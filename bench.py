@@ -50,7 +50,7 @@ def parse_args():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-fp64-reference", action="store_true")
     p.add_argument("--cpu-budget-s", type=float, default=25.0, help="bound on the CPU baseline sample")
-    p.add_argument("--variants", default="split16,fp32,dense16,hi16")
+    p.add_argument("--variants", default="split16,fp32,splitbf16,dense16,hi16")
     p.add_argument("--roofline-only", action="store_true",
                    help="only the standalone interpolation launches and a 64 MiB copy (counter runs)")
     return p.parse_args()
@@ -320,7 +320,7 @@ def main() -> int:
         "data": "synthetic (seeded ellipse phantom, Gaussian coil maps, exact FP64 NUDFT + 40 dB noise)",
         "config": {"workload": "cg-sense 256x256 image / 512x512 grid / 8 coils / golden-angle radial 402x512 / "
                                "KB width 6 / 10 CG / lambda 1e-4",
-                   "samples_per_coil": m, "precision": "fp16 hi+res panels x fp16 2:4 tiles (smfmac), fp32 accumulate, "
+                   "samples_per_coil": fh["output_size"], "precision": "fp16 hi+res panels x fp16 2:4 tiles (smfmac), fp32 accumulate, "
                                                          "fp32 FFT, fp64 CG scalars",
                    "mode": "hipGraph replay of the full reconstruction"},
         "recon_per_s": {v: {"eager": 1000.0 / t["eager_ms"], "graph": 1000.0 / t["graph_ms"]} for v, t in timings.items()},

@@ -121,6 +121,12 @@ int tsn_kb_apodization(uint32_t image0, uint32_t image1, uint32_t grid0, uint32_
  * the reference default seed being 20260823. */
 int tsn_kb_pack(const tsn_kb_matrix* kb, int view, int policy, uint64_t seed, int threads,
                 tsn_host_pack** out);
+/* Tile value formats.  FP16 is the reference format; a BF16 pack has the same
+ * layout and planner, its kept values and metadata chosen on BF16-rounded
+ * coefficients (no counterpart in the reference; used by TSN_PREC_SPLITBF16). */
+enum { TSN_VALUES_FP16 = 0, TSN_VALUES_BF16 = 1 };
+int tsn_kb_pack_values(const tsn_kb_matrix* kb, int view, int policy, uint64_t seed, int threads,
+                       int values, tsn_host_pack** out);
 void tsn_kb_free(tsn_kb_matrix* kb);
 
 /* ------------------------------------------------------- device directions */
@@ -134,8 +140,9 @@ int tsn_direction_load(const char* real_dir, const char* imag_dir, tsn_direction
 int tsn_direction_upload_csr(const tsn_kb_matrix* kb, int view, tsn_direction** out);
 int tsn_direction_header(const tsn_direction* dir, tsn_pack_header* out);
 /* Re-cut the work schedule of a packed direction: regular units are runs of
- * whole groups holding at most unit_tiles tiles each (0 = default, one round
- * of resident waves).  Synchronizes the device; no counterpart in the
+ * whole groups holding at most unit_tiles tiles each (0 = default: 1.5 rounds
+ * of resident waves, 2..8 tiles).  Earlier schedules stay valid for graphs
+ * captured before the call.  Synchronizes the device; no counterpart in the
  * reference (its launch maps one group per warp), used by the autotuner. */
 int tsn_direction_schedule(tsn_direction* dir, uint32_t unit_tiles);
 /* Tile budget of the current schedule (0 for CSR directions). */
@@ -166,12 +173,14 @@ int tsn_split_panel(const void* in, uint32_t rows, uint32_t coils, float scale, 
                     void* stream);
 /* Interleaved split panel: the layout plans use internally (no counterpart in
  * the reference, which keeps hi and residual in two panels).  uint32 per
- * channel = fp16 hi | fp16 residual << 16, blocks of [rows][16] for
- * ceil(coils/8) coil blocks; one 64-byte gather per column instead of two.
- * tsn_split_panel_il rounds exactly as tsn_split_panel; tsn_interp_il is
- * tsn_interp on such a panel (same tiles, same sums, same output). */
-int tsn_split_panel_il(const void* in, uint32_t rows, uint32_t coils, float scale, void* panel,
-                       void* stream);
+ * channel = hi | residual << 16, blocks of [rows][16] for ceil(coils/8) coil
+ * blocks; one 64-byte gather per column instead of two.  values selects FP16
+ * halves (TSN_VALUES_FP16, rounded exactly as tsn_split_panel) or bfloat16
+ * halves (TSN_VALUES_BF16, round to nearest even).  tsn_interp_il is
+ * tsn_interp on such a panel with the direction's own value format (same
+ * tiles, same sums, same output for FP16). */
+int tsn_split_panel_il(const void* in, uint32_t rows, uint32_t coils, float scale, int values,
+                       void* panel, void* stream);
 int tsn_interp_il(const tsn_direction* dir, const void* panel, void* out, uint32_t coils,
                   int conjugate, int accumulate, float scale, int out_layout, void* stream);
 

@@ -592,13 +592,36 @@ def compress_tile(values: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     return comp, pair_metadata(row_meta)
 
 
-def compress_tiles_vectorized(values: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
-    """[T, 16, 32] float64 -> (f16[T,16,16], u32[T,16]) (reference :302-324)."""
+def bf16_bits(values: np.ndarray) -> np.ndarray:
+    """uint16 bfloat16 bits of float32(values), round to nearest even (NaN kept
+    quiet).  Not a reference function: the BF16 tile variant of this build."""
+
+    u = np.asarray(values, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    nan = (u & 0x7FFFFFFF) > 0x7F800000
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF
+    r = np.where(nan, (u >> 16) | 0x40, r)
+    return r.astype(np.uint16)
+
+
+def bf16_decode(bits: np.ndarray) -> np.ndarray:
+    return (np.asarray(bits, dtype=np.uint16).astype(np.uint32) << np.uint32(16)).view(np.float32)
+
+
+def compress_tiles_vectorized(values: np.ndarray, fmt: str = "fp16") -> tuple[np.ndarray, np.ndarray]:
+    """[T, 16, 32] float64 -> (f16[T,16,16], u32[T,16]) (reference :302-324).
+
+    fmt "bf16" applies the same selection to BF16-rounded values and returns
+    the kept values as uint16 BF16 bits."""
 
     t = values.shape[0]
-    quads = values.astype(np.float16).reshape(t, M_TILE, K_TILE // 4, 4)
+    if fmt == "bf16":
+        quads = bf16_bits(values.astype(np.float32)).reshape(t, M_TILE, K_TILE // 4, 4)
+        nonzero = (quads & 0x7FFF) != 0  # -0 counts as zero, as for FP16
+    else:
+        quads = values.astype(np.float16).reshape(t, M_TILE, K_TILE // 4, 4)
+        nonzero = quads != 0
     pos = np.arange(4, dtype=np.uint32)
-    priority = np.where(quads != 0, pos, pos + 4)
+    priority = np.where(nonzero, pos, pos + 4)
     sel = np.sort(np.argsort(priority, axis=-1)[..., :2], axis=-1).astype(np.uint32)
     comp = np.take_along_axis(quads, sel, axis=-1).reshape(t, M_TILE, K_COMP)
     codes = sel[..., 0] | (sel[..., 1] << np.uint32(2))
@@ -628,7 +651,7 @@ def expand_tiles(comp: np.ndarray, pair_meta: np.ndarray) -> np.ndarray:
 
     t = comp.shape[0]
     rm = decode_row_metadata(pair_meta)
-    dense = np.zeros((t, M_TILE, K_TILE), dtype=np.float16)
+    dense = np.zeros((t, M_TILE, K_TILE), dtype=comp.dtype)
     ti = np.arange(t)[:, None]
     ri = np.arange(M_TILE)[None, :]
     for ch in range(K_TILE // 4):
@@ -689,26 +712,26 @@ def header_bytes(output_size: int, n_cols: int, active_rows: int, nnz: int, grou
 
 
 def export_arrays(structure_view: SparseRows, structure_tiles: list[Tile], offsets, row_ids,
-                  component: SparseRows) -> dict:
+                  component: SparseRows, fmt: str = "fp16") -> dict:
     """Arrays of one production component directory (reference :387-436)."""
 
     tiles = materialize_component_tiles(structure_tiles, component)
     col_ids = np.stack([t.col_ids for t in tiles]).astype(np.int32)
     col_ids[col_ids < 0] = 0
-    comp, meta = compress_tiles_vectorized(np.stack([t.values for t in tiles]))
+    comp, meta = compress_tiles_vectorized(np.stack([t.values for t in tiles]), fmt)
     return {
         "header.bin": header_bytes(component.output_size, component.n_cols, component.n_active_rows,
                                    component.nnz, len(offsets) - 1, len(tiles)),
         "group_offsets.bin": np.ascontiguousarray(offsets, dtype=np.uint32),
         "group_row_ids.bin": np.ascontiguousarray(row_ids, dtype=np.int32),
         "tile_col_ids.bin": col_ids,
-        "tile_a_comp.bin": comp.astype(np.float16),
+        "tile_a_comp.bin": comp if fmt == "bf16" else comp.astype(np.float16),
         "tile_pair_meta.bin": meta.astype(np.uint32),
     }
 
 
 def pack_components(real: SparseRows, imag: SparseRows, policy: str = "morton",
-                    seed: int = 20260823) -> dict:
+                    seed: int = 20260823, fmt: str = "fp16") -> dict:
     """Full production pack of one direction: {"real": arrays, "imag": arrays}.
 
     The structure view carries ones in place of the values, as the reference
@@ -720,8 +743,8 @@ def pack_components(real: SparseRows, imag: SparseRows, policy: str = "morton",
     offsets, row_ids, legal, _, res_cols, _ = split_legal_tiles_and_residual(structure, tiles, "split24")
     if len(res_cols):
         raise RuntimeError("split24 structure export left residual entries")
-    return {"real": export_arrays(structure, legal, offsets, row_ids, real),
-            "imag": export_arrays(structure, legal, offsets, row_ids, imag)}
+    return {"real": export_arrays(structure, legal, offsets, row_ids, real, fmt),
+            "imag": export_arrays(structure, legal, offsets, row_ids, imag, fmt)}
 
 
 if __name__ == "__main__":  # pragma: no cover

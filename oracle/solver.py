@@ -33,25 +33,29 @@ def _f16(bits: np.ndarray) -> np.ndarray:
     return np.asarray(bits, dtype=np.uint16).view(np.float16).astype(np.float64)
 
 
-def expand_pack(arrays: dict) -> tuple[np.ndarray, np.ndarray]:
-    """(A_re, A_im) dense float64 [T, 16, 32] from pack arrays."""
+def expand_pack(arrays: dict, values: str = "fp16") -> tuple[np.ndarray, np.ndarray]:
+    """(A_re, A_im) dense float64 [T, 16, 32] from pack arrays (FP16 or BF16 tile values)."""
 
     t = len(arrays["tile_col_ids"]) // 32
-    re = layout.expand_tiles(np.asarray(arrays["real_comp"], np.uint16).view(np.float16).reshape(t, 16, 16),
-                             np.asarray(arrays["real_meta"], np.uint32).reshape(t, 16))
-    im = layout.expand_tiles(np.asarray(arrays["imag_comp"], np.uint16).view(np.float16).reshape(t, 16, 16),
-                             np.asarray(arrays["imag_meta"], np.uint32).reshape(t, 16))
-    return re.astype(np.float64), im.astype(np.float64)
+
+    def dense(comp, meta):
+        bits = layout.expand_tiles(np.asarray(comp, np.uint16).reshape(t, 16, 16),
+                                   np.asarray(meta, np.uint32).reshape(t, 16))
+        vals = layout.bf16_decode(bits) if values == "bf16" else bits.view(np.float16)
+        return vals.astype(np.float64)
+
+    return dense(arrays["real_comp"], arrays["real_meta"]), dense(arrays["imag_comp"], arrays["imag_meta"])
 
 
 def apply_pack(arrays: dict, n_out: int, panel_hi: np.ndarray, panel_res: np.ndarray | None, coils: int,
-               conjugate: bool = False, scale: float = 1.0) -> np.ndarray:
+               conjugate: bool = False, scale: float = 1.0, values: str = "fp16") -> np.ndarray:
     """Packed interpolation in float64 -> complex128 [coils, n_out].
 
-    panels: float16 [blocks, n_cols, 16] (channels 0..7 Re, 8..15 Im of the
-    block's eight coils); rows absent from the pack stay zero."""
+    panels: [blocks, n_cols, 16] real values (channels 0..7 Re, 8..15 Im of
+    the block's eight coils); rows absent from the pack stay zero.  values:
+    the tiles' format (FP16 reference packs or BF16 packs)."""
 
-    a_re, a_im = expand_pack(arrays)
+    a_re, a_im = expand_pack(arrays, values)
     cols = np.asarray(arrays["tile_col_ids"], np.int64).reshape(-1, 32)
     offsets = np.asarray(arrays["group_offsets"], np.int64)
     rows = np.asarray(arrays["group_row_ids"], np.int64).reshape(-1, 16)
@@ -96,11 +100,18 @@ def apply_csr_adjoint(cols: np.ndarray, re: np.ndarray, im: np.ndarray, y: np.nd
 
 
 def components(values: np.ndarray, mode: str) -> list[np.ndarray]:
-    """dense_components / sparse_components value rounding (float32 base)."""
+    """dense_components / sparse_components value rounding (float32 base);
+    "bf16" / "bf16split" are this build's BF16 variant (round to nearest even)."""
 
     v = np.asarray(values, dtype=np.float32)
     if mode == "fp32":
         return [v.astype(np.float64)]
+    if mode in ("bf16", "bf16split"):
+        hb = layout.bf16_decode(layout.bf16_bits(v))
+        if mode == "bf16":
+            return [hb.astype(np.float64)]
+        lb = layout.bf16_decode(layout.bf16_bits(v - hb))
+        return [hb.astype(np.float64), lb.astype(np.float64)]
     hi = v.astype(np.float16).astype(np.float32)
     if mode == "fp16":
         return [hi.astype(np.float64)]

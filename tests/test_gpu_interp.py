@@ -214,3 +214,44 @@ def test_interleaved_panel_path_is_bitwise_the_split_path(view, coils):
     assert np.array_equal(out.cpu().numpy().view(np.uint32), out_il.cpu().numpy().view(np.uint32))
     device.interp_il(d, il, out_il, coils=coils, conjugate=conj, scale=0.25, accumulate=True)
     np.testing.assert_allclose(out_il.cpu().numpy(), 2 * out.cpu().numpy(), rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("view", ["G", "G_T"])
+@pytest.mark.parametrize("coils", [8, 16])
+def test_bf16_tiles_and_panels_match_oracle(view, coils):
+    """BF16 variant: BF16 tiles x BF16 hi + residual panels on the bf16 smfmac,
+    against the float64 product of the same BF16 values; the split panel is
+    bitwise the round-to-nearest-even split."""
+
+    from oracle import layout
+    from oracle.solver import apply_pack
+    from trajsparsenufft_amd import device
+    from trajsparsenufft_amd.device import Direction
+    _, packs = small_case()
+    bp = packs.with_values("bf16")
+    hp = bp.forward if view == "G" else bp.adjoint
+    d = Direction.upload(hp)
+    header, arrays = hp.header, hp.arrays()
+    rng = np.random.default_rng(coils)
+    v = (rng.standard_normal((coils, header["n_cols"])) + 1j * rng.standard_normal((coils, header["n_cols"]))
+         ).astype(np.complex64)
+    il = device.split_panel_il(dev(v), coils, scale=0.5, values="bf16").cpu().numpy().view(np.uint32)
+    nb = (coils + 7) // 8
+    chan = np.zeros((nb, header["n_cols"], 16), np.float32)
+    for c in range(coils):
+        chan[c // 8, :, c % 8] = (v[c] * np.float32(0.5)).real
+        chan[c // 8, :, c % 8 + 8] = (v[c] * np.float32(0.5)).imag
+    hb = layout.bf16_bits(chan)
+    lb = layout.bf16_bits(chan - layout.bf16_decode(hb))
+    assert np.array_equal(il & 0xFFFF, hb) and np.array_equal(il >> 16, lb)
+    conj = view == "G_T"
+    out = torch.zeros((coils, header["output_size"]), dtype=torch.complex64, device="cuda")
+    device.interp_il(d, torch.from_numpy(il.view(np.int32)).cuda(), out, coils=coils, conjugate=conj)
+    ref = apply_pack(arrays, header["output_size"], layout.bf16_decode(hb), layout.bf16_decode(lb), coils, conj,
+                     values="bf16")
+    assert rel_l2(out.cpu().numpy(), ref) <= TOL
+    # a BF16 direction refuses the FP16 entry points, and an FP16 one the BF16 panel path stays FP16
+    from trajsparsenufft_amd._lib import TsnError
+    with pytest.raises(TsnError):
+        device.interp(d, dev(np.zeros((nb, header["n_cols"], 16), np.float16)),
+                      dev(np.zeros((nb, header["n_cols"], 16), np.float16)), out, coils=coils)

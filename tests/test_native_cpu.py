@@ -36,8 +36,9 @@ def _radial(spokes, readout):
     return synth.radial(spokes, readout, True)
 
 
-@pytest.mark.parametrize("image,spokes,readout", [(32, 16, 64), (48, 20, 96)])
-def test_native_packer_bitwise_equals_oracle_planner(image, spokes, readout):
+@pytest.mark.parametrize("image,spokes,readout,values", [(32, 16, 64, "fp16"), (48, 20, 96, "fp16"),
+                                                         (32, 16, 64, "bf16")])
+def test_native_packer_bitwise_equals_oracle_planner(image, spokes, readout, values):
     from trajsparsenufft_amd import pack
     traj = _radial(spokes, readout)
     k = 2 * image
@@ -48,8 +49,8 @@ def test_native_packer_bitwise_equals_oracle_planner(image, spokes, readout):
     imag = layout.sparse_rows_from_ell("I", cols, im_, k * k, coords)
     views = {"G": (real, imag), "G_T": (layout.transpose_sparse(real, k), layout.transpose_sparse(imag, k))}
     for view, (vr, vi) in views.items():
-        ref = layout.pack_components(vr, vi)
-        hp = kb.pack(view, threads=2)
+        ref = layout.pack_components(vr, vi, fmt=values)
+        hp = kb.pack(view, threads=2, values=values)
         got = hp.arrays()
         hdr = hp.header
         for comp in ("real", "imag"):

@@ -97,3 +97,21 @@ def test_split2_components_bitwise():
     np.testing.assert_array_equal(lo, (v - v.astype(np.float16).astype(np.float32)).astype(np.float16).astype(np.float64))
     # hi + lo keeps ~22 bits; residuals below the FP16 normal range keep 2^-24 absolute
     assert np.all(np.abs(hi + lo - v) <= 2.5e-7 * np.abs(v) + 2 ** -25)
+
+
+def test_bf16_rounding_matches_torch_and_definition():
+    """bfloat16 round-to-nearest-even of the BF16 variant, checked against
+    torch's float32 -> bfloat16 conversion and the halfway/tie cases."""
+
+    torch = pytest.importorskip("torch")
+    from oracle import layout
+    rng = np.random.default_rng(8)
+    v = np.concatenate([rng.standard_normal(20000).astype(np.float32) * 10.0 ** rng.integers(-30, 30, 20000),
+                        np.array([0.0, -0.0, 1.0, -2.5, 3.4e38, 1e-40, np.inf, -np.inf], np.float32)]).astype(np.float32)
+    ref = torch.from_numpy(v).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    np.testing.assert_array_equal(layout.bf16_bits(v), ref)
+    # ties: 1 + 2^-8 lies halfway between bf16 neighbours 1 and 1 + 2^-7 -> even (1.0)
+    tie = np.array([1 + 2 ** -8, 1 + 3 * 2 ** -8], np.float32)
+    np.testing.assert_array_equal(layout.bf16_decode(layout.bf16_bits(tie)), [1.0, 1 + 2 ** -6])
+    hi, lo = solver.components(v[:20000], "bf16split")
+    assert np.all(np.abs(hi + lo - v[:20000]) <= 2.0 ** -16 * np.abs(v[:20000]))

@@ -14,6 +14,7 @@ LIBRARY_PATH = Path(__file__).resolve().parent / "libtsn.so"
 
 TSN_OK, TSN_ERR_RUNTIME, TSN_ERR_INCOMPATIBLE, TSN_ERR_CG, TSN_ERR_ARGUMENT = 0, 1, 2, 3, 4
 PREC_SPLIT16, PREC_DENSE16, PREC_FP32, PREC_HI16, PREC_SPLITBF16 = 0, 1, 2, 3, 4
+VALUES_FP16, VALUES_BF16 = 0, 1
 OUT_COIL_MAJOR, OUT_ROW_MAJOR = 0, 1
 
 
@@ -56,6 +57,7 @@ SIGNATURES = {
     "tsn_kb_arrays": (C.c_int, [P, C.POINTER(U32), C.POINTER(U32), PP, PP, PP]),
     "tsn_kb_apodization": (C.c_int, [U32, U32, U32, U32, U32, P]),
     "tsn_kb_pack": (C.c_int, [P, C.c_int, C.c_int, C.c_uint64, C.c_int, PP]),
+    "tsn_kb_pack_values": (C.c_int, [P, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int, PP]),
     "tsn_kb_free": (None, [P]),
     "tsn_direction_upload": (C.c_int, [P, PP]),
     "tsn_direction_load": (C.c_int, [C.c_char_p, C.c_char_p, PP]),
@@ -69,7 +71,7 @@ SIGNATURES = {
     "tsn_interp_dense": (C.c_int, [P, P, P, P, U32, C.c_int, C.c_int, C.c_float, C.c_int, P]),
     "tsn_interp_f32": (C.c_int, [P, P, P, U32, C.c_int, C.c_int, C.c_float, C.c_int, P]),
     "tsn_split_panel": (C.c_int, [P, U32, U32, C.c_float, P, P, P]),
-    "tsn_split_panel_il": (C.c_int, [P, U32, U32, C.c_float, P, P]),
+    "tsn_split_panel_il": (C.c_int, [P, U32, U32, C.c_float, C.c_int, P, P]),
     "tsn_interp_il": (C.c_int, [P, P, P, U32, C.c_int, C.c_int, C.c_float, C.c_int, P]),
     "tsn_plan_create": (C.c_int, [C.POINTER(PlanConfig), P, P, P, P, PP]),
     "tsn_plan_free": (None, [P]),

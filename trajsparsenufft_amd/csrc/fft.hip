@@ -27,11 +27,27 @@ namespace tsn {
 template <int R>
 __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs a) {
   extern __shared__ float2 lds[];
+  __shared__ double red[512];
   constexpr int K = 64 * R;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t n1 = blockIdx.x, chunk = blockIdx.y;
   const uint32_t c = chunk * 8 + w;
   const bool active = c < a.coils;
+  double beta = 0.0;
+  if (a.cg_r) {
+    // every block reduces the same partials in the same order: same beta
+    double s = 0.0;
+    for (uint32_t i = threadIdx.x; i < a.rr_count; i += blockDim.x) s += a.rr_partial[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 256; off > 0; off >>= 1) {
+      if (static_cast<int>(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
+      __syncthreads();
+    }
+    const double rho_new = red[0];
+    beta = rho_new / fmax(a.rho_hist[a.iter - 1], 1e-30);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.rho_hist[a.iter] = rho_new;
+  }
   float2 x[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) x[j] = make_float2(0.f, 0.f);
@@ -40,7 +56,15 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs a) {
     const uint32_t n0 = lane + 64 * j;
     if (active && n0 < a.n0) {
       const size_t pix = static_cast<size_t>(n1) * a.n0 + n0;
-      const float2 v = cmul(a.image_t[pix], a.smaps_t[(static_cast<size_t>(n1) * a.coils + c) * a.n0 + n0]);
+      float2 img;
+      if (a.cg_r) {
+        const float2 rv = a.cg_r[pix], pv = a.cg_p_old[pix];
+        img = make_float2(static_cast<float>(rv.x + beta * pv.x), static_cast<float>(rv.y + beta * pv.y));
+        if (chunk == 0 && w == 0) a.cg_p_new[pix] = img;
+      } else {
+        img = a.image_t[pix];
+      }
+      const float2 v = cmul(img, a.smaps_t[(static_cast<size_t>(n1) * a.coils + c) * a.n0 + n0]);
       const float s = a.apod_t[pix];
       x[j] = make_float2(v.x * s, v.y * s);
     }
@@ -54,20 +78,22 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs a) {
   const uint32_t cw = min(8u, a.coils - chunk * 8);
   for (uint32_t k = threadIdx.x; k < static_cast<uint32_t>(K); k += blockDim.x) {
     float2* dst = a.t_fwd + (static_cast<size_t>(k) * a.n1 + n1) * a.coils + chunk * 8;
-    const float2* src = lds + k * 8;
     if (cw == 8) {
       float4* d4 = reinterpret_cast<float4*>(dst);
-      const float4* s4 = reinterpret_cast<const float4*>(src);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) d4[i] = s4[i];
+      for (int i = 0; i < 4; ++i) {
+        const float2 u = lds[xslot(k, 2 * i)], v = lds[xslot(k, 2 * i + 1)];
+        d4[i] = make_float4(u.x, u.y, v.x, v.y);
+      }
     } else {
-      for (uint32_t i = 0; i < cw; ++i) dst[i] = src[i];
+      for (uint32_t i = 0; i < cw; ++i) dst[i] = lds[xslot(k, i)];
     }
   }
 }
 
 // Forward, second axis: grid row k0 -> panel rows k0*K1 + k1.  OUT 0: FP16
-// hi/residual N16 panel blocks; OUT 1: float2 [row][coils].
+// hi/residual interleaved N16 panel blocks; OUT 1: float2 [row][coils];
+// OUT 2: BF16 hi/residual interleaved panel blocks.
 template <int R, int OUT>
 __global__ __launch_bounds__(512) void fwd_axis1_kernel(FftArgs a) {
   extern __shared__ float2 lds[];
@@ -77,15 +103,15 @@ __global__ __launch_bounds__(512) void fwd_axis1_kernel(FftArgs a) {
   const uint32_t cw = min(8u, a.coils - chunk * 8);
   for (uint32_t e = threadIdx.x; e < a.n1 * 8; e += blockDim.x) {
     const uint32_t n1 = e >> 3, c = e & 7;
-    lds[e] = c < cw ? a.t_fwd[(static_cast<size_t>(k0) * a.n1 + n1) * a.coils + chunk * 8 + c]
-                    : make_float2(0.f, 0.f);
+    lds[xslot(n1, c)] = c < cw ? a.t_fwd[(static_cast<size_t>(k0) * a.n1 + n1) * a.coils + chunk * 8 + c]
+                               : make_float2(0.f, 0.f);
   }
   __syncthreads();
   float2 x[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const uint32_t n1 = lane + 64 * j;
-    x[j] = n1 < a.n1 ? lds[n1 * 8 + w] : make_float2(0.f, 0.f);
+    x[j] = n1 < a.n1 ? lds[xslot(n1, w)] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   constexpr int TPL = (R * 8 + 63) / 64;
@@ -97,26 +123,30 @@ __global__ __launch_bounds__(512) void fwd_axis1_kernel(FftArgs a) {
   const uint64_t grid_rows = static_cast<uint64_t>(a.k0) * K;
   for (uint32_t k1 = threadIdx.x; k1 < static_cast<uint32_t>(K); k1 += blockDim.x) {
     const uint64_t r = static_cast<uint64_t>(k0) * K + k1;
-    const float2* src = lds + k1 * 8;
-    if constexpr (OUT == 0) {
+    if constexpr (OUT == 0 || OUT == 2) {
       uint32_t w[16];  // channel c = Re coil c, c + 8 = Im coil c; hi | res << 16
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
-        const float2 v = src[c];
-        _Float16 h, r2;
-        split16(v.x, h, r2);
-        w[c] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
-               (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r2)) << 16);
-        split16(v.y, h, r2);
-        w[c + 8] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
-                   (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r2)) << 16);
+        const float2 v = lds[xslot(k1, c)];
+        if constexpr (OUT == 2) {
+          w[c] = split_bf16(v.x);
+          w[c + 8] = split_bf16(v.y);
+        } else {
+          _Float16 h, r2;
+          split16(v.x, h, r2);
+          w[c] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
+                 (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r2)) << 16);
+          split16(v.y, h, r2);
+          w[c + 8] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
+                     (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r2)) << 16);
+        }
       }
       uint4* d4 = reinterpret_cast<uint4*>(a.panel_il + (chunk * grid_rows + r) * 16);
 #pragma unroll
       for (int i = 0; i < 4; ++i) d4[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
     } else {
       float2* dst = a.panel32 + r * a.coils + chunk * 8;
-      for (uint32_t i = 0; i < cw; ++i) dst[i] = src[i];
+      for (uint32_t i = 0; i < cw; ++i) dst[i] = lds[xslot(k1, i)];
     }
   }
 }
@@ -132,13 +162,13 @@ __global__ __launch_bounds__(512) void inv_axis1_kernel(FftArgs a) {
   const uint32_t cw = min(8u, a.coils - chunk * 8);
   for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(K) * 8; e += blockDim.x) {
     const uint32_t k1 = e >> 3, c = e & 7;
-    lds[e] = c < cw ? a.grid[(static_cast<size_t>(k0) * K + k1) * a.coils + chunk * 8 + c]
-                    : make_float2(0.f, 0.f);
+    lds[xslot(k1, c)] = c < cw ? a.grid[(static_cast<size_t>(k0) * K + k1) * a.coils + chunk * 8 + c]
+                               : make_float2(0.f, 0.f);
   }
   __syncthreads();
   float2 x[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) x[j] = lds[(lane + 64 * j) * 8 + w];
+  for (int j = 0; j < R; ++j) x[j] = lds[xslot(lane + 64 * j, w)];
   __syncthreads();
   constexpr int TPL = (R * 8 + 63) / 64;
   float2 out[TPL][8];
@@ -148,14 +178,15 @@ __global__ __launch_bounds__(512) void inv_axis1_kernel(FftArgs a) {
   __syncthreads();
   for (uint32_t n1 = threadIdx.x; n1 < a.n1; n1 += blockDim.x) {
     float2* dst = a.t_inv + (static_cast<size_t>(n1) * a.k0 + k0) * a.coils + chunk * 8;
-    const float2* src = lds + n1 * 8;
     if (cw == 8) {
       float4* d4 = reinterpret_cast<float4*>(dst);
-      const float4* s4 = reinterpret_cast<const float4*>(src);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) d4[i] = s4[i];
+      for (int i = 0; i < 4; ++i) {
+        const float2 u = lds[xslot(n1, 2 * i)], v = lds[xslot(n1, 2 * i + 1)];
+        d4[i] = make_float4(u.x, u.y, v.x, v.y);
+      }
     } else {
-      for (uint32_t i = 0; i < cw; ++i) dst[i] = src[i];
+      for (uint32_t i = 0; i < cw; ++i) dst[i] = lds[xslot(n1, i)];
     }
   }
 }
@@ -176,13 +207,13 @@ __global__ __launch_bounds__(512) void inv_axis0_kernel(FftArgs a) {
     const uint32_t cw = min(8u, a.coils - chunk * 8);
     for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(K) * 8; e += blockDim.x) {
       const uint32_t k0 = e >> 3, c = e & 7;
-      lds[e] = c < cw ? a.t_inv[(static_cast<size_t>(n1) * K + k0) * a.coils + chunk * 8 + c]
-                      : make_float2(0.f, 0.f);
+      lds[xslot(k0, c)] = c < cw ? a.t_inv[(static_cast<size_t>(n1) * K + k0) * a.coils + chunk * 8 + c]
+                                 : make_float2(0.f, 0.f);
     }
     __syncthreads();
     float2 x[R];
 #pragma unroll
-    for (int j = 0; j < R; ++j) x[j] = lds[(lane + 64 * j) * 8 + w];
+    for (int j = 0; j < R; ++j) x[j] = lds[xslot(lane + 64 * j, w)];
     __syncthreads();
     constexpr int TPL = (R * 8 + 63) / 64;
     float2 out[TPL][8];
@@ -205,11 +236,11 @@ __global__ __launch_bounds__(512) void inv_axis0_kernel(FftArgs a) {
     emit_exchange<R, true>(out, lds, lane, w);
     __syncthreads();
     if (threadIdx.x < a.n0) {
-      const float2* src = lds + threadIdx.x * 8;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        acc.x += src[i].x;
-        acc.y += src[i].y;
+      for (int i = 0; i < 8; ++i) {  // coil order
+        const float2 v = lds[xslot(threadIdx.x, i)];
+        acc.x += v.x;
+        acc.y += v.y;
       }
     }
     __syncthreads();
@@ -273,6 +304,9 @@ int fft_forward(const FftArgs& a, int out_kind, hipStream_t s) {
                                         lds_bytes_for<R>(), s, a));
   if (out_kind == 0) {
     TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 0>), dim3(a.k0, chunks), dim3(512),
+                                          lds_bytes_for<R>(), s, a));
+  } else if (out_kind == 2) {
+    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 2>), dim3(a.k0, chunks), dim3(512),
                                           lds_bytes_for<R>(), s, a));
   } else {
     TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 1>), dim3(a.k0, chunks), dim3(512),

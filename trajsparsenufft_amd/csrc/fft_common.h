@@ -27,7 +27,25 @@ struct FftArgs {
   float lambda;
   float out_scale;
   double* dot_partial;     // [n1] or null
+  // CG direction update fused into the forward first axis (cg_r null: the
+  // input is image_t).  Input p_new = r + beta p_old, beta = rho_new /
+  // max(rho_old, 1e-30), rho_new = fixed-order sum of rr_partial; rho values
+  // live in rho_hist (rho_hist[iter - 1] read, rho_hist[iter] written by
+  // block (0, 0)); coil chunk 0 stores p_new.
+  const float2* cg_r;
+  const float2* cg_p_old;
+  float2* cg_p_new;
+  const double* rr_partial;
+  uint32_t rr_count;
+  double* rho_hist;
+  uint32_t iter;
 };
+
+// bfloat16 hi | residual << 16 of v (both round to nearest even).
+__device__ __forceinline__ uint32_t split_bf16(float v) {
+  const uint16_t h = bf16_rne_bits(v);
+  return static_cast<uint32_t>(h) | (static_cast<uint32_t>(bf16_rne_bits(v - bf16_to_float(h))) << 16);
+}
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -197,7 +215,14 @@ __device__ __forceinline__ void wave_fft(float2 (&x)[R], float2* buf, const floa
   (void)HALF_OUT;
 }
 
-// Write the wave's outputs to the block exchange area lds[k*8 + wave]
+// Block exchange area: row k holds one element of each of the block's 8
+// waves (coils).  Element c of row k sits in slot c ^ ((k ^ k >> 3) & 7), so
+// the three access patterns (a wave walking rows, a thread walking the 8
+// elements of its row, contiguous fills from global memory) stay at the
+// minimum 2-way bank sharing instead of up to 16-way conflicts.
+__device__ __forceinline__ int xslot(int k, int c) { return k * 8 + (c ^ ((k ^ (k >> 3)) & 7)); }
+
+// Write the wave's outputs to the block exchange area, element w of row k
 // (k < K, or k < K/2 when HALF_OUT).  Call between two __syncthreads.
 template <int R, bool HALF_OUT>
 __device__ __forceinline__ void emit_exchange(const float2 (&out)[(R * 8 + 63) / 64][8], float2* lds,
@@ -211,7 +236,7 @@ __device__ __forceinline__ void emit_exchange(const float2 (&out)[(R * 8 + 63) /
 #pragma unroll
     for (int k2b = 0; k2b < (HALF_OUT ? 4 : 8); ++k2b) {
       const int k = k1 + R * (k2a + 8 * k2b);
-      lds[k * 8 + w] = out[s][k2b];
+      lds[xslot(k, w)] = out[s][k2b];
     }
   }
 }

@@ -54,13 +54,15 @@ __global__ __launch_bounds__(512) void compat_fwd_rows(const float2* __restrict_
   float2 out[1][8];
   wave_fft<kR, false, true, false>(x, lds + w * wave_lds_elems<kR>(), g_twiddle512, lane, out);
   __syncthreads();
-  emit_exchange<kR, false>(out, lds, lane, w);  // lds[kx*8 + row]
+  emit_exchange<kR, false>(out, lds, lane, w);  // element row of exchange row kx
   __syncthreads();
   const int kx = threadIdx.x;
   float4* dst = reinterpret_cast<float4*>(ws + (static_cast<size_t>(c) * kG + kx) * kG + blockIdx.x * 8);
-  const float4* src = reinterpret_cast<const float4*>(lds + kx * 8);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) dst[i] = src[i];
+  for (int i = 0; i < 4; ++i) {
+    const float2 u = lds[xslot(kx, 2 * i)], v = lds[xslot(kx, 2 * i + 1)];
+    dst[i] = make_float4(u.x, u.y, v.x, v.y);
+  }
 }
 
 // column kx of all coils along y -> N16 hi [+ residual] panel [+ fp32 debug]
@@ -78,14 +80,14 @@ __global__ __launch_bounds__(512) void compat_fwd_cols(const float2* __restrict_
   float2 out[1][8];
   wave_fft<kR, false, true, false>(x, lds + w * wave_lds_elems<kR>(), g_twiddle512, lane, out);
   __syncthreads();
-  emit_exchange<kR, false>(out, lds, lane, w);  // lds[ky*8 + c]
+  emit_exchange<kR, false>(out, lds, lane, w);  // element c of exchange row ky
   __syncthreads();
   const int ky = threadIdx.x;
   const size_t r = static_cast<size_t>(ky) * kG + kx;
   half8 hre, him, rre, rim;
 #pragma unroll
   for (int cc = 0; cc < kC; ++cc) {
-    const float2 v = lds[ky * 8 + cc];
+    const float2 v = lds[xslot(ky, cc)];
     _Float16 h, q;
     split16(v.x, h, q);
     hre[cc] = h;
@@ -118,14 +120,16 @@ __global__ __launch_bounds__(512) void compat_inv_rows(const float2* __restrict_
   float2 out[1][8];
   wave_fft<kR, true, false, true>(x, lds + w * wave_lds_elems<kR>(), g_twiddle512, lane, out);
   __syncthreads();
-  emit_exchange<kR, true>(out, lds, lane, w);  // lds[x*8 + row]
+  emit_exchange<kR, true>(out, lds, lane, w);  // element row of exchange row x
   __syncthreads();
   if (threadIdx.x < kI) {
     const int xi = threadIdx.x;
     float4* dst = reinterpret_cast<float4*>(ws + (static_cast<size_t>(c) * kG + xi) * kG + blockIdx.x * 8);
-    const float4* src = reinterpret_cast<const float4*>(lds + xi * 8);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) dst[i] = src[i];
+    for (int i = 0; i < 4; ++i) {
+      const float2 u = lds[xslot(xi, 2 * i)], v = lds[xslot(xi, 2 * i + 1)];
+      dst[i] = make_float4(u.x, u.y, v.x, v.y);
+    }
   }
 }
 
@@ -143,12 +147,12 @@ __global__ __launch_bounds__(512) void compat_inv_cols(const float2* __restrict_
   float2 out[1][8];
   wave_fft<kR, true, false, true>(x, lds + w * wave_lds_elems<kR>(), g_twiddle512, lane, out);
   __syncthreads();
-  emit_exchange<kR, true>(out, lds, lane, w);  // lds[y*8 + col]
+  emit_exchange<kR, true>(out, lds, lane, w);  // element col of exchange row y
   __syncthreads();
   for (int e = threadIdx.x; e < kI * 8; e += blockDim.x) {
     const int y = e >> 3, xi = e & 7;
     const float s = scaling[y * kI + x0 + xi] * (1.0f / static_cast<float>(kG * kG));
-    const float2 v = lds[y * 8 + xi];
+    const float2 v = lds[xslot(y, xi)];
     image[(static_cast<size_t>(c) * kI + y) * kI + x0 + xi] = make_float2(v.x * s, v.y * s);
   }
 }

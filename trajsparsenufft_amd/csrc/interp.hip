@@ -23,12 +23,16 @@ using namespace tsn;
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 namespace tsn {
 
-enum OutMode { kOutCoilMajor = 0, kOutRowMajor = 1, kOutSplit16 = 2, kOutSplit16IL = 3 };
-enum TileKind { kSparseSplit = 0, kDenseSplit = 1, kSparseHi = 2 };
+enum OutMode { kOutCoilMajor = 0, kOutRowMajor = 1, kOutSplit16 = 2, kOutSplit16IL = 3, kOutSplitBF16IL = 4 };
+// Tile kinds: FP16 tiles x FP16 hi + residual on smfmac, the same on dense
+// MFMA, FP16 hi only, and BF16 tiles x BF16 hi + residual on smfmac.
+enum TileKind { kSparseSplit = 0, kDenseSplit = 1, kSparseHi = 2, kSparseSplitBF16 = 3 };
 
 // Block b runs on XCD b % 8; give each XCD a contiguous range of items.
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
@@ -79,7 +83,12 @@ __device__ __forceinline__ void interp_epilogue(const InterpArgs& a, const int4&
     v *= a.scale;
     const int row = rr[i];
     if (row < 0) continue;
-    if constexpr (OM == kOutSplit16 || OM == kOutSplit16IL) {
+    if constexpr (OM == kOutSplitBF16IL) {
+      const uint16_t hb = bf16_rne_bits(v);
+      const uint16_t rb = bf16_rne_bits(v - bf16_to_float(hb));
+      a.out_il[(static_cast<size_t>(b) * a.n_out + row) * kN + n] =
+          static_cast<uint32_t>(hb) | (static_cast<uint32_t>(rb) << 16);
+    } else if constexpr (OM == kOutSplit16 || OM == kOutSplit16IL) {
       const _Float16 hi = static_cast<_Float16>(v);
       const _Float16 res = static_cast<_Float16>(v - static_cast<float>(hi));
       const size_t o = (static_cast<size_t>(b) * a.n_out + row) * kN + n;
@@ -240,6 +249,16 @@ __device__ __forceinline__ void mma_tile(const TileOps<NB, IL>& o, float4_t (&ac
       acc_im[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(dim, hi, acc_im[b], 0, 0, 0);
       acc_re[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(dre, res, acc_re[b], 0, 0, 0);
       acc_im[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(dim, res, acc_im[b], 0, 0, 0);
+    } else if constexpr (KIND == kSparseSplitBF16) {
+      const bf16x4_t are = __builtin_bit_cast(bf16x4_t, make_uint2(o.a0.x, o.a0.y));
+      const bf16x4_t aim = __builtin_bit_cast(bf16x4_t, make_uint2(o.a0.z, o.a0.w));
+      const bf16x8_t bhi = __builtin_bit_cast(bf16x8_t, hi);
+      const bf16x8_t bres = __builtin_bit_cast(bf16x8_t, res);
+      const int idx = static_cast<int>(o.idx);
+      acc_re[b] = __builtin_amdgcn_smfmac_f32_16x16x32_bf16(are, bhi, acc_re[b], idx, 0, 0);
+      acc_im[b] = __builtin_amdgcn_smfmac_f32_16x16x32_bf16(aim, bhi, acc_im[b], idx, 0, 1);
+      acc_re[b] = __builtin_amdgcn_smfmac_f32_16x16x32_bf16(are, bres, acc_re[b], idx, 0, 0);
+      acc_im[b] = __builtin_amdgcn_smfmac_f32_16x16x32_bf16(aim, bres, acc_im[b], idx, 0, 1);
     } else {
       const half4_t are = __builtin_bit_cast(half4_t, make_uint2(o.a0.x, o.a0.y));
       const half4_t aim = __builtin_bit_cast(half4_t, make_uint2(o.a0.z, o.a0.w));
@@ -520,7 +539,9 @@ __global__ __launch_bounds__(256) void split_panel_kernel(const float2* __restri
   res[gtid] = static_cast<_Float16>(v - static_cast<float>(h));
 }
 
-// Interleaved variant for the plan: one uint32 = hi | res << 16 per channel.
+// Interleaved variant for the plan: one uint32 = hi | res << 16 per channel,
+// FP16 (RN) or, with BF16, bfloat16 (RNE) halves.
+template <bool BF16>
 __global__ __launch_bounds__(256) void split_panel_il_kernel(const float2* __restrict__ in, uint32_t rows,
                                                              uint32_t coils, uint32_t blocks, float scale,
                                                              uint32_t* __restrict__ out) {
@@ -537,10 +558,15 @@ __global__ __launch_bounds__(256) void split_panel_il_kernel(const float2* __res
     const float2 x = in[static_cast<size_t>(coil) * rows + row];
     v = (n < 8 ? x.x : x.y) * scale;
   }
-  const _Float16 h = static_cast<_Float16>(v);
-  const _Float16 r = static_cast<_Float16>(v - static_cast<float>(h));
-  out[gtid] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
-              (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r)) << 16);
+  if constexpr (BF16) {
+    const uint16_t hb = bf16_rne_bits(v);
+    out[gtid] = static_cast<uint32_t>(hb) | (static_cast<uint32_t>(bf16_rne_bits(v - bf16_to_float(hb))) << 16);
+  } else {
+    const _Float16 h = static_cast<_Float16>(v);
+    const _Float16 r = static_cast<_Float16>(v - static_cast<float>(h));
+    out[gtid] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
+                (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r)) << 16);
+  }
 }
 
 // ------------------------------------------------------------- dispatch
@@ -550,7 +576,7 @@ static void launch_tiles_om(const InterpArgs& a, bool conj, bool acc, hipStream_
   const uint32_t blocks = a.n_long + div_up(a.n_items, kTileWaves);
   if (!blocks) return;
   const dim3 grid(blocks), block(kTileWaves * 64);
-  constexpr bool kAccOk = OM != kOutSplit16 && OM != kOutSplit16IL;
+  constexpr bool kAccOk = OM != kOutSplit16 && OM != kOutSplit16IL && OM != kOutSplitBF16IL;
   if (conj) {
     if (kAccOk && acc) hipLaunchKernelGGL((interp_tiles_kernel<KIND, NB, IL, OM, true, kAccOk>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((interp_tiles_kernel<KIND, NB, IL, OM, true, false>), grid, block, 0, s, a);
@@ -568,6 +594,7 @@ static void launch_tiles_nb(const InterpArgs& a, int om, bool conj, bool acc, hi
   if constexpr (IL) {
     if (om == kOutCoilMajor) launch_tiles_om<KIND, NB, IL, kOutCoilMajor>(a, conj, acc, s);
     else if (om == kOutRowMajor) launch_tiles_om<KIND, NB, IL, kOutRowMajor>(a, conj, acc, s);
+    else if (KIND == kSparseSplitBF16) launch_tiles_om<KIND, NB, IL, kOutSplitBF16IL>(a, conj, false, s);
     else launch_tiles_om<KIND, NB, IL, kOutSplit16IL>(a, conj, false, s);
   } else {
     if (om == kOutCoilMajor) launch_tiles_om<KIND, NB, IL, kOutCoilMajor>(a, conj, acc, s);
@@ -579,6 +606,11 @@ static void launch_tiles_nb(const InterpArgs& a, int om, bool conj, bool acc, hi
 template <int KIND>
 static void launch_tiles(const InterpArgs& a, bool il, int om, bool conj, bool acc, hipStream_t s) {
   const bool two = a.coils > kCoilsPerBlock;
+  if constexpr (KIND == kSparseSplitBF16) {  // interleaved panels only (checked by launch_interp)
+    if (two) launch_tiles_nb<KIND, 2, true>(a, om, conj, acc, s);
+    else launch_tiles_nb<KIND, 1, true>(a, om, conj, acc, s);
+    return;
+  }
   if (il) {
     if (two) launch_tiles_nb<KIND, 2, true>(a, om, conj, acc, s);
     else launch_tiles_nb<KIND, 1, true>(a, om, conj, acc, s);
@@ -606,6 +638,10 @@ int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* 
     return fail(TSN_ERR_ARGUMENT, "output mode %d does not match the panel layout", out_mode);
   if (kind == kDenseSplit && !d->tile_dense)
     return fail(TSN_ERR_ARGUMENT, "direction has no dense-control tiles");
+  if ((kind == kSparseSplitBF16) != (d->values == TSN_VALUES_BF16))
+    return fail(TSN_ERR_ARGUMENT, "tile value format of the direction does not match the interpolation variant");
+  if (kind == kSparseSplitBF16 && !interleaved)
+    return fail(TSN_ERR_ARGUMENT, "BF16 tiles take an interleaved split panel");
   InterpArgs a{};
   a.items = d->items;
   a.n_items = d->n_items;
@@ -632,6 +668,8 @@ int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* 
   a.out_il = out_il;
   if (kind == kSparseSplit) launch_tiles<kSparseSplit>(a, interleaved, out_mode, conj, accumulate, stream);
   else if (kind == kDenseSplit) launch_tiles<kDenseSplit>(a, interleaved, out_mode, conj, accumulate, stream);
+  else if (kind == kSparseSplitBF16)
+    launch_tiles<kSparseSplitBF16>(a, interleaved, out_mode, conj, accumulate, stream);
   else launch_tiles<kSparseHi>(a, interleaved, out_mode, conj, accumulate, stream);
   TSN_HIP(hipGetLastError());
   return TSN_OK;
@@ -689,14 +727,20 @@ int launch_split_panel(const float2* in, uint32_t rows, uint32_t coils, float sc
 }
 
 int launch_split_panel_il(const float2* in, uint32_t rows, uint32_t coils, float scale, uint32_t* out,
-                          hipStream_t stream) {
+                          hipStream_t stream, int values) {
   if (!in || !out || rows == 0) return fail(TSN_ERR_ARGUMENT, "bad split arguments");
   if (coils == 0 || coils > static_cast<uint32_t>(kMaxCoils))
     return fail(TSN_ERR_ARGUMENT, "coils must be in 1..16 (got %u)", coils);
+  if (values != TSN_VALUES_FP16 && values != TSN_VALUES_BF16)
+    return fail(TSN_ERR_ARGUMENT, "unknown value format %d", values);
   const uint32_t blocks = div_up(coils, kCoilsPerBlock);
   const uint64_t total = static_cast<uint64_t>(blocks) * rows * kN;
-  hipLaunchKernelGGL(split_panel_il_kernel, dim3(div_up(total, 256)), dim3(256), 0, stream, in, rows,
-                     coils, blocks, scale, out);
+  if (values == TSN_VALUES_BF16)
+    hipLaunchKernelGGL(split_panel_il_kernel<true>, dim3(div_up(total, 256)), dim3(256), 0, stream, in, rows,
+                       coils, blocks, scale, out);
+  else
+    hipLaunchKernelGGL(split_panel_il_kernel<false>, dim3(div_up(total, 256)), dim3(256), 0, stream, in, rows,
+                       coils, blocks, scale, out);
   TSN_HIP(hipGetLastError());
   return TSN_OK;
 }
@@ -858,6 +902,7 @@ static int upload_pack(const HostPack& p, tsn_direction** out) {
   }
   auto* d = new tsn_direction();
   d->h = h;
+  d->values = p.values;
   (void)hipGetDevice(&d->device);
   d->host_offsets = p.group_offsets;
   size_t bytes = 0;
@@ -1032,15 +1077,16 @@ int tsn_interp_il(const tsn_direction* d, const void* panel, void* out, uint32_t
                   int accumulate, float scale, int out_layout, void* stream) {
   if (out_layout != TSN_OUT_COIL_MAJOR && out_layout != TSN_OUT_ROW_MAJOR)
     return fail(TSN_ERR_ARGUMENT, "unknown output layout %d", out_layout);
-  return launch_interp(d, kSparseSplit, panel, nullptr, coils, out_layout, conjugate != 0, accumulate != 0,
-                       scale, static_cast<float*>(out), nullptr, nullptr, static_cast<hipStream_t>(stream),
-                       true, nullptr);
+  if (!d) return fail(TSN_ERR_ARGUMENT, "null direction");
+  return launch_interp(d, d->values == TSN_VALUES_BF16 ? kSparseSplitBF16 : kSparseSplit, panel, nullptr, coils,
+                       out_layout, conjugate != 0, accumulate != 0, scale, static_cast<float*>(out), nullptr,
+                       nullptr, static_cast<hipStream_t>(stream), true, nullptr);
 }
 
-int tsn_split_panel_il(const void* in, uint32_t rows, uint32_t coils, float scale, void* panel,
+int tsn_split_panel_il(const void* in, uint32_t rows, uint32_t coils, float scale, int values, void* panel,
                        void* stream) {
   return launch_split_panel_il(static_cast<const float2*>(in), rows, coils, scale,
-                               static_cast<uint32_t*>(panel), static_cast<hipStream_t>(stream));
+                               static_cast<uint32_t*>(panel), static_cast<hipStream_t>(stream), values);
 }
 
 int tsn_split_panel(const void* in, uint32_t rows, uint32_t coils, float scale, void* hi, void* res,

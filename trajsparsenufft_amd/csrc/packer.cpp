@@ -480,7 +480,7 @@ inline float lookup_value(const View& v, const std::vector<float>& vals, int32_t
 
 // compress one component of one tile: kept values + pair metadata
 void compress(const View& v, const std::vector<float>& vals, const int32_t* local_rows, const Tile& t,
-              uint16_t* comp, uint32_t* pair_meta) {
+              bool bf16, uint16_t* comp, uint32_t* pair_meta) {
   uint32_t row_meta[kM];
   for (int r = 0; r < kM; ++r) {
     const int32_t lr = local_rows[r];
@@ -491,7 +491,7 @@ void compress(const View& v, const std::vector<float>& vals, const int32_t* loca
         const int j = 4 * q + p;
         float val = 0.0f;
         if (lr >= 0 && ((t.rowmask[r] >> j) & 1)) val = lookup_value(v, vals, lr, t.cols[j]);
-        h[p] = half_bits(val);
+        h[p] = bf16 ? bf16_rne_bits(val) : half_bits(val);
       }
       int keep[2], nk = 0;
       for (int p = 0; p < 4 && nk < 2; ++p)
@@ -514,7 +514,7 @@ void compress(const View& v, const std::vector<float>& vals, const int32_t* loca
   }
 }
 
-int pack_view(const View& v, int policy, uint64_t seed, int threads, HostPack& out) {
+int pack_view(const View& v, int policy, uint64_t seed, int threads, bool bf16, HostPack& out) {
   const uint32_t n = static_cast<uint32_t>(v.row_ids.size());
   std::vector<int32_t> order(n);
   std::iota(order.begin(), order.end(), 0);
@@ -551,8 +551,8 @@ int pack_view(const View& v, int policy, uint64_t seed, int threads, HostPack& o
     size_t t = out.group_offsets[g];
     for (const Tile& tile : results[g].tiles) {
       for (int j = 0; j < kK; ++j) out.tile_col_ids[t * kK + j] = tile.cols[j] >= 0 ? tile.cols[j] : 0;
-      compress(v, v.re, lr, tile, out.real_comp.data() + t * 256, out.real_meta.data() + t * 16);
-      compress(v, v.im, lr, tile, out.imag_comp.data() + t * 256, out.imag_meta.data() + t * 16);
+      compress(v, v.re, lr, tile, bf16, out.real_comp.data() + t * 256, out.real_meta.data() + t * 16);
+      compress(v, v.im, lr, tile, bf16, out.imag_comp.data() + t * 256, out.imag_meta.data() + t * 16);
       ++t;
     }
   });
@@ -675,19 +675,26 @@ int tsn_kb_apodization(uint32_t image0, uint32_t image1, uint32_t grid0, uint32_
   return TSN_OK;
 }
 
-int tsn_kb_pack(const tsn_kb_matrix* kb, int view, int policy, uint64_t seed, int threads, tsn_host_pack** out) {
-  if (!kb || !out || (view != 0 && view != 1) || (policy != 0 && policy != 1))
+int tsn_kb_pack_values(const tsn_kb_matrix* kb, int view, int policy, uint64_t seed, int threads, int values,
+                       tsn_host_pack** out) {
+  if (!kb || !out || (view != 0 && view != 1) || (policy != 0 && policy != 1) ||
+      (values != TSN_VALUES_FP16 && values != TSN_VALUES_BF16))
     return fail(TSN_ERR_ARGUMENT, "bad arguments");
   *out = nullptr;
   const View v = view == 0 ? forward_view(*kb) : transpose_view(*kb);
   auto* p = new tsn_host_pack();
-  const int rc = pack_view(v, policy, seed, threads, *p);
+  p->values = values;
+  const int rc = pack_view(v, policy, seed, threads, values == TSN_VALUES_BF16, *p);
   if (rc != TSN_OK) {
     delete p;
     return rc;
   }
   *out = p;
   return TSN_OK;
+}
+
+int tsn_kb_pack(const tsn_kb_matrix* kb, int view, int policy, uint64_t seed, int threads, tsn_host_pack** out) {
+  return tsn_kb_pack_values(kb, view, policy, seed, threads, TSN_VALUES_FP16, out);
 }
 
 void tsn_kb_free(tsn_kb_matrix* kb) { delete kb; }

@@ -8,7 +8,10 @@
 //   inverse FFT * apod * sqrt(K0K1)/(K0K1)        :335-352 (prenormalised)
 //   normal = combine * 1/s^2 + lambda/(K0K1)/s^2  :27-54, 401-404
 //   CG update order, curvature flag (alpha = 0)   :56-180, 241-268
-// Differences by design: float64 CG scalars (rho, <p,Ap>, alpha, beta, history);
+// Differences by design: float64 CG scalars (rho, <p,Ap>, alpha, beta, history),
+// reduced in fixed order by every block that needs them, so an iteration is
+// seven kernels (the direction update rides in the first forward FFT pass,
+// alpha in the x/r update);
 // image vectors kept column-major [n1][n0] inside the plan; SENSE combine,
 // Tikhonov term and the <p, Ap> partial fused into the last inverse FFT pass;
 // the forward interpolation writes the adjoint's FP16 hi/res sample panels
@@ -24,7 +27,7 @@ int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* 
                   _Float16* out_hi, _Float16* out_res, hipStream_t stream, bool interleaved,
                   uint32_t* out_il);
 int launch_split_panel_il(const float2* in, uint32_t rows, uint32_t coils, float scale, uint32_t* out,
-                          hipStream_t stream);
+                          hipStream_t stream, int values);
 int launch_interp_csr(const tsn_direction* d, const float2* in, uint32_t coils, int out_mode,
                       bool conj, bool accumulate, float scale, float2* out, hipStream_t stream);
 int launch_split_panel(const float2* in, uint32_t rows, uint32_t coils, float scale, _Float16* hi,
@@ -36,14 +39,6 @@ using namespace tsn;
 namespace {
 
 constexpr int kVecThreads = 256;
-constexpr int kVecPerThread = 4;
-
-struct CgScalars {
-  double rho;
-  double denominator;
-  double alpha;
-  double beta;
-};
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
   red[threadIdx.x] = v;
@@ -55,6 +50,13 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return red[0];
 }
 
+// Fixed-order sum of partials: every block that calls it gets the same value.
+__device__ __forceinline__ double sum_partials(const double* __restrict__ partial, uint32_t count, double* red) {
+  double s = 0.0;
+  for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) s += partial[i];
+  return block_sum(s, red);
+}
+
 // x = 0, r = p = b, partial <r, r>
 __global__ __launch_bounds__(kVecThreads) void cg_init_kernel(const float2* __restrict__ b,
                                                               float2* __restrict__ x, float2* __restrict__ r,
@@ -62,97 +64,64 @@ __global__ __launch_bounds__(kVecThreads) void cg_init_kernel(const float2* __re
                                                               double* __restrict__ partial) {
   __shared__ double red[kVecThreads];
   double s = 0.0;
-  const uint32_t base = blockIdx.x * kVecThreads * kVecPerThread + threadIdx.x;
-#pragma unroll
-  for (int i = 0; i < kVecPerThread; ++i) {
-    const uint32_t e = base + i * kVecThreads;
-    if (e < n) {
-      const float2 v = b[e];
-      x[e] = make_float2(0.f, 0.f);
-      r[e] = v;
-      p[e] = v;
-      s += static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y;
-    }
+  const uint32_t e = blockIdx.x * kVecThreads + threadIdx.x;
+  if (e < n) {
+    const float2 v = b[e];
+    x[e] = make_float2(0.f, 0.f);
+    r[e] = v;
+    p[e] = v;
+    s = static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y;
   }
   s = block_sum(s, red);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// x += alpha p ; r -= alpha Ap ; partial <r, r>
+// alpha = rho_k / Re<p, Ap> (0 and status 0 on non-positive or non-finite
+// curvature), x += alpha p, r -= alpha Ap, partial <r, r>.  rho_k is
+// rho_hist[iter]; every block reduces the same <p, Ap> partials itself.
 __global__ __launch_bounds__(kVecThreads) void cg_update_xr_kernel(
     float2* __restrict__ x, float2* __restrict__ r, const float2* __restrict__ p,
-    const float2* __restrict__ ap, const CgScalars* __restrict__ sc, uint32_t n,
-    double* __restrict__ partial) {
+    const float2* __restrict__ ap, const double* __restrict__ dot_partial, uint32_t dot_count,
+    const double* __restrict__ rho_hist, uint32_t iter, int* __restrict__ status, uint32_t n,
+    double* __restrict__ rr_partial) {
   __shared__ double red[kVecThreads];
-  const double alpha = sc->alpha;
+  const double den = sum_partials(dot_partial, dot_count, red);
+  __syncthreads();
+  double alpha = 0.0;
+  if (isfinite(den) && den > 0.0) alpha = rho_hist[iter] / den;
+  else if (blockIdx.x == 0 && threadIdx.x == 0) *status = 0;
   double s = 0.0;
-  const uint32_t base = blockIdx.x * kVecThreads * kVecPerThread + threadIdx.x;
-#pragma unroll
-  for (int i = 0; i < kVecPerThread; ++i) {
-    const uint32_t e = base + i * kVecThreads;
-    if (e < n) {
-      const float2 pv = p[e], av = ap[e];
-      float2 xv = x[e], rv = r[e];
-      xv.x = static_cast<float>(xv.x + alpha * pv.x);
-      xv.y = static_cast<float>(xv.y + alpha * pv.y);
-      rv.x = static_cast<float>(rv.x - alpha * av.x);
-      rv.y = static_cast<float>(rv.y - alpha * av.y);
-      x[e] = xv;
-      r[e] = rv;
-      s += static_cast<double>(rv.x) * rv.x + static_cast<double>(rv.y) * rv.y;
-    }
+  const uint32_t e = blockIdx.x * kVecThreads + threadIdx.x;
+  if (e < n) {
+    const float2 pv = p[e], av = ap[e];
+    float2 xv = x[e], rv = r[e];
+    xv.x = static_cast<float>(xv.x + alpha * pv.x);
+    xv.y = static_cast<float>(xv.y + alpha * pv.y);
+    rv.x = static_cast<float>(rv.x - alpha * av.x);
+    rv.y = static_cast<float>(rv.y - alpha * av.y);
+    x[e] = xv;
+    r[e] = rv;
+    s = static_cast<double>(rv.x) * rv.x + static_cast<double>(rv.y) * rv.y;
   }
   s = block_sum(s, red);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  if (threadIdx.x == 0) rr_partial[blockIdx.x] = s;
 }
 
-// p = r + beta p
-__global__ __launch_bounds__(kVecThreads) void cg_update_p_kernel(float2* __restrict__ p,
-                                                                  const float2* __restrict__ r,
-                                                                  const CgScalars* __restrict__ sc,
-                                                                  uint32_t n) {
-  const double beta = sc->beta;
-  const uint32_t base = blockIdx.x * kVecThreads * kVecPerThread + threadIdx.x;
-#pragma unroll
-  for (int i = 0; i < kVecPerThread; ++i) {
-    const uint32_t e = base + i * kVecThreads;
-    if (e < n) {
-      const float2 rv = r[e], pv = p[e];
-      p[e] = make_float2(static_cast<float>(rv.x + beta * pv.x), static_cast<float>(rv.y + beta * pv.y));
-    }
-  }
-}
-
-enum ReduceMode { kReduceInit = 0, kReduceAlpha = 1, kReduceBeta = 2 };
-
-// One block sums the partials in a fixed order, then updates the scalars.
-__global__ __launch_bounds__(kVecThreads) void cg_reduce_kernel(const double* __restrict__ partial,
-                                                                uint32_t count, int mode, int iteration,
-                                                                CgScalars* __restrict__ sc,
-                                                                double* __restrict__ history,
-                                                                int* __restrict__ status) {
+// rho_hist[index] = sum of partials (status = 1 when index == 0).  With
+// finish, rho_hist[0 .. index] then become the residual history sqrt(rho).
+__global__ __launch_bounds__(kVecThreads) void cg_rho_kernel(const double* __restrict__ partial,
+                                                             uint32_t count, double* __restrict__ rho_hist,
+                                                             uint32_t index, int finish,
+                                                             int* __restrict__ status) {
   __shared__ double red[kVecThreads];
-  double s = 0.0;
-  for (uint32_t i = threadIdx.x; i < count; i += kVecThreads) s += partial[i];
-  s = block_sum(s, red);
-  if (threadIdx.x != 0) return;
-  if (mode == kReduceInit) {
-    sc->rho = s;
-    history[0] = sqrt(fmax(s, 0.0));
-    *status = 1;
-  } else if (mode == kReduceAlpha) {
-    sc->denominator = s;
-    if (!isfinite(s) || s <= 0.0) {
-      *status = 0;
-      sc->alpha = 0.0;
-    } else {
-      sc->alpha = sc->rho / s;
-    }
-  } else {
-    sc->beta = s / fmax(sc->rho, 1e-30);
-    sc->rho = s;
-    history[iteration + 1] = sqrt(fmax(s, 0.0));
+  const double s = sum_partials(partial, count, red);
+  if (threadIdx.x == 0) {
+    rho_hist[index] = s;
+    if (index == 0) *status = 1;
   }
+  if (!finish) return;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i <= index; i += blockDim.x) rho_hist[i] = sqrt(fmax(rho_hist[i], 0.0));
 }
 
 // [rows][cols] -> [cols][rows] complex, 32x32 tiles, optional scale.
@@ -230,9 +199,8 @@ struct tsn_plan {
   float* apod_t = nullptr;
   uint32_t *pan_il = nullptr, *smp_il = nullptr;  // interleaved hi | res << 16 panels
   float2 *pan32 = nullptr, *smp32 = nullptr, *grid = nullptr;
-  float2 *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *b = nullptr, *tmp = nullptr;
-  double* partial = nullptr;
-  CgScalars* scalars = nullptr;
+  float2 *x = nullptr, *r = nullptr, *p = nullptr, *p2 = nullptr, *ap = nullptr, *b = nullptr, *tmp = nullptr;
+  double *partial = nullptr, *rr_partial = nullptr;  // <p, Ap> per image column; <r, r> per vector block
   int* status = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -280,11 +248,15 @@ FftArgs fft_args(const tsn_plan* pl) {
 }
 
 bool is_fp32(const tsn_plan* pl) { return pl->cfg.precision == TSN_PREC_FP32; }
+bool is_bf16(const tsn_plan* pl) { return pl->cfg.precision == TSN_PREC_SPLITBF16; }
+// forward FFT output: 0 FP16 interleaved split panel, 1 FP32 rows, 2 BF16 interleaved split panel
+int fft_out_kind(const tsn_plan* pl) { return is_fp32(pl) ? 1 : is_bf16(pl) ? 2 : 0; }
 
 int interp_kind(const tsn_plan* pl) {
   switch (pl->cfg.precision) {
     case TSN_PREC_DENSE16: return 1;
     case TSN_PREC_HI16: return 2;
+    case TSN_PREC_SPLITBF16: return 3;
     default: return 0;
   }
 }
@@ -297,7 +269,7 @@ float grid_norm(const tsn_plan* pl) {
 int op_forward_grid(tsn_plan* pl, const float2* image_t, hipStream_t s) {
   FftArgs a = fft_args(pl);
   a.image_t = image_t;
-  return fft_forward(a, is_fp32(pl) ? 1 : 0, s);
+  return fft_forward(a, fft_out_kind(pl), s);
 }
 
 // panels -> samples: split panels (internal) or float2 output (coil-major)
@@ -352,7 +324,8 @@ int op_load_samples(tsn_plan* pl, const float2* samples, float scale, hipStream_
     TSN_HIP(hipGetLastError());
     return TSN_OK;
   }
-  return launch_split_panel_il(samples, pl->m, pl->cfg.coils, scale, pl->smp_il, s);
+  return launch_split_panel_il(samples, pl->m, pl->cfg.coils, scale, pl->smp_il, s,
+                               is_bf16(pl) ? TSN_VALUES_BF16 : TSN_VALUES_FP16);
 }
 
 int transpose_image(const float2* in, uint32_t rows, uint32_t cols, float scale, float2* out, hipStream_t s) {
@@ -374,20 +347,37 @@ int enqueue_cg(tsn_plan* pl, const float2* measurement, uint32_t iterations, flo
   TSN_CHECK(op_adjoint_grid(pl, s));
   TSN_CHECK(op_adjoint_image(pl, pl->b, nullptr, 0.0f, inv_scale_sq, nullptr, s));
   hipLaunchKernelGGL(cg_init_kernel, dim3(pl->vec_blocks), dim3(kVecThreads), 0, s, pl->b, pl->x, pl->r,
-                     pl->p, n, pl->partial);
-  hipLaunchKernelGGL(cg_reduce_kernel, dim3(1), dim3(kVecThreads), 0, s, pl->partial, pl->vec_blocks,
-                     static_cast<int>(kReduceInit), 0, pl->scalars, history, pl->status);
+                     pl->p, n, pl->rr_partial);
+  hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kVecThreads), 0, s, pl->rr_partial, pl->vec_blocks, history,
+                     0u, 0, pl->status);
+  // rho_k lives in history[k] until the end.  Iteration k: the first forward
+  // FFT pass forms p_k = r + beta p_{k-1} (k > 0, ping-pong buffers) and
+  // rho_k; the last inverse pass forms <p_k, A p_k>; one kernel applies alpha.
+  float2* p_cur = pl->p;
+  float2* p_other = pl->p2;
   for (uint32_t it = 0; it < iterations; ++it) {
-    TSN_CHECK(op_normal(pl, pl->p, pl->ap, inv_scale_sq, scaled_lambda, pl->partial, s));
-    hipLaunchKernelGGL(cg_reduce_kernel, dim3(1), dim3(kVecThreads), 0, s, pl->partial, pl->cfg.image1,
-                       static_cast<int>(kReduceAlpha), static_cast<int>(it), pl->scalars, history, pl->status);
+    FftArgs a = fft_args(pl);
+    if (it == 0) {
+      a.image_t = p_cur;
+    } else {
+      a.cg_r = pl->r;
+      a.cg_p_old = p_cur;
+      a.cg_p_new = p_other;
+      a.rr_partial = pl->rr_partial;
+      a.rr_count = pl->vec_blocks;
+      a.rho_hist = history;
+      a.iter = it;
+      std::swap(p_cur, p_other);
+    }
+    TSN_CHECK(fft_forward(a, fft_out_kind(pl), s));
+    TSN_CHECK(op_forward_samples(pl, nullptr, s));
+    TSN_CHECK(op_adjoint_grid(pl, s));
+    TSN_CHECK(op_adjoint_image(pl, pl->ap, p_cur, scaled_lambda, inv_scale_sq, pl->partial, s));
     hipLaunchKernelGGL(cg_update_xr_kernel, dim3(pl->vec_blocks), dim3(kVecThreads), 0, s, pl->x, pl->r,
-                       pl->p, pl->ap, pl->scalars, n, pl->partial);
-    hipLaunchKernelGGL(cg_reduce_kernel, dim3(1), dim3(kVecThreads), 0, s, pl->partial, pl->vec_blocks,
-                       static_cast<int>(kReduceBeta), static_cast<int>(it), pl->scalars, history, pl->status);
-    hipLaunchKernelGGL(cg_update_p_kernel, dim3(pl->vec_blocks), dim3(kVecThreads), 0, s, pl->p, pl->r,
-                       pl->scalars, n);
+                       p_cur, pl->ap, pl->partial, pl->cfg.image1, history, it, pl->status, n, pl->rr_partial);
   }
+  hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kVecThreads), 0, s, pl->rr_partial, pl->vec_blocks, history,
+                     iterations, 1, pl->status);
   TSN_CHECK(transpose_image(pl->x, pl->cfg.image1, pl->cfg.image0, 1.0f, x_out, s));
   hipLaunchKernelGGL(copy_status_kernel, dim3(1), dim3(64), 0, s, pl->status, status_out);
   TSN_HIP(hipGetLastError());
@@ -435,8 +425,8 @@ int tsn_plan_create(const tsn_plan_config* cfg, const tsn_direction* fwd, const 
   if (c.grid0 != 2 * c.image0 || c.grid1 != 2 * c.image1 || !fft_length_supported(c.grid0) ||
       !fft_length_supported(c.grid1))
     return fail(TSN_ERR_ARGUMENT, "grid must be 2x image with grid sides in {64, 128, 256, 512, 768}");
-  if (c.precision < TSN_PREC_SPLIT16 || c.precision > TSN_PREC_HI16)
-    return fail(TSN_ERR_ARGUMENT, "precision %d is not available in this build", c.precision);
+  if (c.precision < TSN_PREC_SPLIT16 || c.precision > TSN_PREC_SPLITBF16)
+    return fail(TSN_ERR_ARGUMENT, "unknown precision %d", c.precision);
   const bool fp32 = c.precision == TSN_PREC_FP32;
   if (fwd->csr != fp32 || adj->csr != fp32)
     return fail(TSN_ERR_ARGUMENT, "FP32 plans need CSR directions, FP16 plans need packed directions");
@@ -445,6 +435,12 @@ int tsn_plan_create(const tsn_plan_config* cfg, const tsn_direction* fwd, const 
     return fail(TSN_ERR_INCOMPATIBLE, "unexpected operator shape");
   if (c.precision == TSN_PREC_DENSE16 && (!fwd->tile_dense || !adj->tile_dense))
     return fail(TSN_ERR_ARGUMENT, "dense control needs dense tiles");
+  if (!fp32) {
+    const int want = c.precision == TSN_PREC_SPLITBF16 ? TSN_VALUES_BF16 : TSN_VALUES_FP16;
+    if (fwd->values != want || adj->values != want)
+      return fail(TSN_ERR_ARGUMENT, "precision %d needs %s packs", c.precision,
+                  want == TSN_VALUES_BF16 ? "BF16" : "FP16");
+  }
   auto* pl = new tsn_plan();
   pl->cfg = c;
   (void)hipGetDevice(&pl->device);
@@ -453,7 +449,7 @@ int tsn_plan_create(const tsn_plan_config* cfg, const tsn_direction* fwd, const 
   pl->m = fwd->h.output_size;
   pl->blocks = div_up(c.coils, kCoilsPerBlock);
   pl->pixels = c.image0 * c.image1;
-  pl->vec_blocks = div_up(pl->pixels, kVecThreads * kVecPerThread);
+  pl->vec_blocks = div_up(pl->pixels, kVecThreads);
   const size_t px = pl->pixels;
   int rc = TSN_OK;
   auto A = [&](auto** p, size_t count) {
@@ -473,9 +469,9 @@ int tsn_plan_create(const tsn_plan_config* cfg, const tsn_direction* fwd, const 
     A(&pl->pan_il, static_cast<size_t>(pl->blocks) * grid * kN);
     A(&pl->smp_il, static_cast<size_t>(pl->blocks) * pl->m * kN);
   }
-  for (float2** v : {&pl->x, &pl->r, &pl->p, &pl->ap, &pl->b, &pl->tmp}) A(v, px);
+  for (float2** v : {&pl->x, &pl->r, &pl->p, &pl->p2, &pl->ap, &pl->b, &pl->tmp}) A(v, px);
   A(&pl->partial, std::max<size_t>(pl->vec_blocks, c.image1));
-  A(&pl->scalars, 1);
+  A(&pl->rr_partial, pl->vec_blocks);
   A(&pl->status, 1);
   if (rc == TSN_OK) {
     const auto t0 = twiddles(c.grid0), t1 = twiddles(c.grid1);

@@ -46,13 +46,26 @@ int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
 
+// bfloat16 bits of v, round to nearest even: the top half of the float32 bits
+// after adding 0x7FFF plus the lowest kept bit; NaN stays a quiet NaN.
+__host__ __device__ inline uint16_t bf16_rne_bits(float v) {
+  uint32_t u = __builtin_bit_cast(uint32_t, v);
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+__host__ __device__ inline float bf16_to_float(uint16_t b) {
+  return __builtin_bit_cast(float, static_cast<uint32_t>(b) << 16);
+}
+
 // One packed direction in host memory (real + imag share the structure).
 struct HostPack {
   tsn_pack_header h{};
   std::vector<uint32_t> group_offsets;  // [groups + 1]
   std::vector<int32_t> group_row_ids;   // [groups * 16]
   std::vector<int32_t> tile_col_ids;    // [tiles * 32]
-  std::vector<uint16_t> real_comp;      // [tiles * 16 * 16] fp16 bits
+  int values = TSN_VALUES_FP16;         // format of the 16-bit tile values (files hold FP16)
+  std::vector<uint16_t> real_comp;      // [tiles * 16 * 16] value bits
   std::vector<uint32_t> real_meta;      // [tiles * 16]
   std::vector<uint16_t> imag_comp;
   std::vector<uint32_t> imag_meta;
@@ -99,6 +112,7 @@ struct tsn_direction {
   tsn_pack_header h{};
   int device = 0;
   bool csr = false;
+  int values = TSN_VALUES_FP16;      // tile value format (TSN_VALUES_*)
   // ---- packed 2:4 tiles (device)
   uint32_t* group_offsets = nullptr;
   int32_t* group_rows = nullptr;     // [groups * 16] logical output rows, -1 = padding

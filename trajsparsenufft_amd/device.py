@@ -22,6 +22,7 @@ import os
 
 import torch
 
+from . import _lib
 from ._lib import OUT_COIL_MAJOR, OUT_ROW_MAJOR, PackHeader, check, lib
 from .pack import HEADER_FIELDS, HostPack, KbMatrix
 
@@ -152,13 +153,15 @@ def split_panel(samples: torch.Tensor, coils: int, scale: float = 1.0):
     return hi, res
 
 
-def split_panel_il(samples: torch.Tensor, coils: int, scale: float = 1.0) -> torch.Tensor:
-    """complex64 [coils, rows] -> interleaved int32 [blocks, rows, 16] (hi | res << 16)."""
+def split_panel_il(samples: torch.Tensor, coils: int, scale: float = 1.0, values: str = "fp16") -> torch.Tensor:
+    """complex64 [coils, rows] -> interleaved int32 [blocks, rows, 16] (hi | res << 16),
+    FP16 halves or (values="bf16") bfloat16 halves."""
 
     rows = samples.numel() // coils
     _require(samples, torch.complex64, coils * rows, "samples")
     panel = torch.empty((coil_blocks(coils), rows, 16), dtype=torch.int32, device=samples.device)
-    check(lib().tsn_split_panel_il(_ptr(samples), rows, coils, float(scale), _ptr(panel), _stream()))
+    fmt = {"fp16": _lib.VALUES_FP16, "bf16": _lib.VALUES_BF16}[values]
+    check(lib().tsn_split_panel_il(_ptr(samples), rows, coils, float(scale), fmt, _ptr(panel), _stream()))
     return panel
 
 

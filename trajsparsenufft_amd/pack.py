@@ -30,6 +30,9 @@ POLICIES = {"morton": 0, "trajectory": 1}
 HEADER_FIELDS = [name for name, _ in PackHeader._fields_]
 
 
+VALUE_FORMATS = {"fp16": _lib.VALUES_FP16, "bf16": _lib.VALUES_BF16}
+
+
 def _header_dict(h: PackHeader) -> dict:
     return {name: int(getattr(h, name)) for name in HEADER_FIELDS}
 
@@ -135,9 +138,13 @@ class KbMatrix:
         return grab(cols, np.int32), grab(re, np.float32), grab(im, np.float32)
 
     def pack(self, view: str = "G", policy: str = "morton", seed: int = DEFAULT_SEED,
-             threads: int = 0) -> HostPack:
+             threads: int = 0, values: str = "fp16") -> HostPack:
+        """Pack one view with the production planner.  values "fp16" is the
+        reference format; "bf16" keeps BF16-rounded coefficients (same planner)."""
+
         out = C.c_void_p()
-        check(lib().tsn_kb_pack(self._h, VIEWS[view], POLICIES[policy], seed, threads, C.byref(out)))
+        check(lib().tsn_kb_pack_values(self._h, VIEWS[view], POLICIES[policy], seed, threads, VALUE_FORMATS[values],
+                                       C.byref(out)))
         return HostPack(out)
 
 
@@ -158,6 +165,17 @@ class TrajectoryPacks:
     forward: HostPack
     adjoint: HostPack
     apod: np.ndarray
+    values: str = "fp16"
+    seed: int = DEFAULT_SEED
+
+    def with_values(self, values: str, threads: int = 0) -> "TrajectoryPacks":
+        """The same trajectory packed with another tile value format."""
+
+        if values == self.values:
+            return self
+        return TrajectoryPacks(self.kb, self.kb.pack("G", "morton", self.seed, threads, values),
+                               self.kb.pack("G_T", "morton", self.seed, threads, values), self.apod, values,
+                               self.seed)
 
     def write(self, root: str | os.PathLike) -> None:
         """Write ``root/{G,G_T}/{real,imag}`` in the reference layout."""
@@ -167,10 +185,12 @@ class TrajectoryPacks:
 
 
 def prepare_trajectory(trajectory: np.ndarray, image_size=(256, 256), grid_size=(512, 512),
-                       numpoints: int = 6, seed: int = DEFAULT_SEED, threads: int = 0) -> TrajectoryPacks:
+                       numpoints: int = 6, seed: int = DEFAULT_SEED, threads: int = 0,
+                       values: str = "fp16") -> TrajectoryPacks:
     kb = KbMatrix.build(trajectory, image_size, grid_size, numpoints, threads)
-    return TrajectoryPacks(kb, kb.pack("G", "morton", seed, threads), kb.pack("G_T", "morton", seed, threads),
-                           apodization(image_size, grid_size, numpoints))
+    return TrajectoryPacks(kb, kb.pack("G", "morton", seed, threads, values),
+                           kb.pack("G_T", "morton", seed, threads, values),
+                           apodization(image_size, grid_size, numpoints), values, seed)
 
 
 __all__ = ["HostPack", "KbMatrix", "TrajectoryPacks", "apodization", "prepare_trajectory", "DEFAULT_SEED"]

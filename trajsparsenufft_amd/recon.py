@@ -18,12 +18,13 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from ._lib import (PREC_DENSE16, PREC_FP32, PREC_HI16, PREC_SPLIT16, TSN_ERR_CG, PlanConfig, TsnError, check,
-                   lib)
+from ._lib import (PREC_DENSE16, PREC_FP32, PREC_HI16, PREC_SPLIT16, PREC_SPLITBF16, TSN_ERR_CG, PlanConfig,
+                   TsnError, check, lib)
 from .device import Direction, _ptr, _require, _stream
 from .pack import TrajectoryPacks, prepare_trajectory
 
-PRECISIONS = {"split16": PREC_SPLIT16, "dense16": PREC_DENSE16, "fp32": PREC_FP32, "hi16": PREC_HI16}
+PRECISIONS = {"split16": PREC_SPLIT16, "dense16": PREC_DENSE16, "fp32": PREC_FP32, "hi16": PREC_HI16,
+              "splitbf16": PREC_SPLITBF16}
 
 
 @dataclass
@@ -163,6 +164,7 @@ def build_plan(packs: TrajectoryPacks, smaps: torch.Tensor, precision: str = "sp
     if precision == "fp32":
         fwd, adj = Direction.csr(packs.kb, "G"), Direction.csr(packs.kb, "G_T")
     else:
+        packs = packs.with_values("bf16" if precision == "splitbf16" else "fp16")
         fwd, adj = Direction.upload(packs.forward), Direction.upload(packs.adjoint)
     apod = torch.from_numpy(np.ascontiguousarray(packs.apod)).to(smaps.device)
     return Plan(fwd, adj, smaps, apod, image_size, grid_size, precision)
