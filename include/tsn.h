@@ -147,6 +147,13 @@ int tsn_direction_header(const tsn_direction* dir, tsn_pack_header* out);
 int tsn_direction_schedule(tsn_direction* dir, uint32_t unit_tiles);
 /* Tile budget of the current schedule (0 for CSR directions). */
 uint32_t tsn_direction_unit_tiles(const tsn_direction* dir);
+/* Autotuner: time the interpolation of this direction for `coils` coils over
+ * the candidate unit sizes {2,3,4,5,6,8,10,12,16} (scratch panels allocated
+ * and freed here, the stream synchronized) and keep the fastest schedule.
+ * Results do not depend on the schedule (bitwise).  No counterpart in the
+ * reference; run once per trajectory and coil count. */
+int tsn_direction_autotune(tsn_direction* dir, uint32_t coils, void* stream, uint32_t* best_unit_tiles,
+                           float* best_us);
 /* Device bytes the direction occupies (pack + index arrays). */
 size_t tsn_direction_bytes(const tsn_direction* dir);
 void tsn_direction_free(tsn_direction* dir);
@@ -184,6 +191,23 @@ int tsn_split_panel_il(const void* in, uint32_t rows, uint32_t coils, float scal
 int tsn_interp_il(const tsn_direction* dir, const void* panel, void* out, uint32_t coils,
                   int conjugate, int accumulate, float scale, int out_layout, void* stream);
 
+/* Data consistency of the forward direction (JKF_DATA_CONSISTENCY_MODE 1/2 of
+ * nufft_integrated_bench.cu:67-82, 497-525, 700-715; epilogue
+ * nufft_spmma_bench.cu:561-603 and integrated_residual_to_half_rows :375-408):
+ *   r[row][c] = (scale * (G panel)[row][c] - measurement_scale * y[row][c]) * density[row]
+ * written as an interleaved split panel of the direction's value format
+ * (the adjoint's input, as tsn_split_panel_il lays it out).  y is float2 in
+ * measurement_layout (TSN_OUT_COIL_MAJOR [coils][rows], the reference's, or
+ * TSN_OUT_ROW_MAJOR [rows][coils], one 64-byte read per row); density float
+ * [rows] or NULL for all ones.  TSN_DC_MATERIALIZED interpolates into
+ * workspace (float2 [rows][coils]) and forms r in a second pass;
+ * TSN_DC_FUSED forms r in the interpolation epilogue.  Both give the same
+ * bits (each step rounds on its own). */
+enum { TSN_DC_MATERIALIZED = 1, TSN_DC_FUSED = 2 };
+int tsn_interp_dc(const tsn_direction* forward, const void* panel, const void* measurements,
+                  int measurement_layout, const float* density, uint32_t coils, float scale,
+                  float measurement_scale, int mode, void* workspace, void* out_panel, void* stream);
+
 /* ------------------------------------------------------------------ plan */
 
 typedef struct tsn_plan_config {
@@ -213,6 +237,18 @@ int tsn_adjoint(tsn_plan* plan, const void* samples, void* image, void* stream);
 int tsn_normal(tsn_plan* plan, const void* image, void* out, float inv_scale_sq,
                float scaled_lambda, void* stream);
 
+/* Data-consistency gradient A^H W (A x - measurement_scale * y) in the
+ * operator normalisation of tsn_forward/tsn_adjoint (pass measurement_scale =
+ * 1/sqrt(grid0*grid1) for raw torchkbnufft-scale data): image and out float2
+ * [image0][image1], y float2 [coils][M], density float [M] or NULL (ones).
+ * Split-panel precisions form the weighted residual in the forward
+ * interpolation epilogue (TSN_DC_FUSED); FP32 in a pass over the CSR output.
+ * The step either side of the forward interpolation in unrolled and
+ * regularised solvers (nufft_integrated_bench.cu:700-715 + the SENSE
+ * combine of nufft_fp16x2_cg_bench.cu:27-54). */
+int tsn_data_consistency(tsn_plan* plan, const void* image, const void* measurement, const float* density,
+                         float measurement_scale, void* out, void* stream);
+
 /* CG-SENSE (launch_cg): measurement raw float2 [coils][M] (torchkbnufft scale),
  * operator_scale as the reference CLI argument (native scale), lambda the raw
  * Tikhonov weight (reference 1e-4).  Writes x float2 [image0][image1], the
@@ -227,6 +263,40 @@ int tsn_cg_graph_capture(tsn_plan* plan, const void* measurement, uint32_t itera
                          float operator_scale, float lambda, void* x_out, void* residuals_out,
                          void* status_out, void* stream);
 int tsn_cg_graph_launch(tsn_plan* plan, void* stream);
+
+/* ------------------------------------------------------------- batches */
+
+/* Many independent CG-SENSE problems of one image / grid shape solved in
+ * lockstep: every step is one launch for all of them (the FFT passes and the
+ * vector kernels index the problem, the interpolation merges all problems'
+ * work units; problems sharing a pack read its tiles once per step).  Each
+ * problem's result equals its tsn_cg result bit for bit.  No counterpart in
+ * the reference, which reconstructs one slice per process; for uneven slice
+ * batches (spokes, coils) and frame series sharing one packed trajectory. */
+typedef struct tsn_batch tsn_batch;
+typedef struct tsn_batch_problem {
+  const tsn_direction* forward; /* G pack (rows = samples) */
+  const tsn_direction* adjoint; /* G_T pack */
+  const void* smaps;            /* device float2 [coils][image0][image1] (copied) */
+  uint32_t coils;               /* 1..16 */
+} tsn_batch_problem;
+/* cfg: image / grid / precision (split16, dense16, hi16 or splitbf16; coils
+ * ignored).  apodization: device float [image0][image1], shared. */
+int tsn_batch_create(const tsn_plan_config* cfg, const tsn_batch_problem* problems, uint32_t count,
+                     const void* apodization, tsn_batch** out);
+void tsn_batch_free(tsn_batch* batch);
+size_t tsn_batch_bytes(const tsn_batch* batch);
+/* Per problem z: measurements[z] raw float2 [coils][M_z], operator_scales[z],
+ * x_out[z] float2 [image0][image1], residuals_out[z] double[iterations + 1];
+ * status_out int[count] (1 ok, 0 non-positive curvature).  The host arrays are
+ * read during the call (device tables are staged after draining the stream). */
+int tsn_batch_cg(tsn_batch* batch, const void* const* measurements, uint32_t iterations,
+                 const float* operator_scales, float lambda, void* const* x_out,
+                 void* const* residuals_out, void* status_out, void* stream);
+int tsn_batch_cg_graph_capture(tsn_batch* batch, const void* const* measurements, uint32_t iterations,
+                               const float* operator_scales, float lambda, void* const* x_out,
+                               void* const* residuals_out, void* status_out, void* stream);
+int tsn_batch_cg_graph_launch(tsn_batch* batch, void* stream);
 
 /* ----------------------------------------------- FFT endpoint drop-in ABI */
 /* Same names and argument meaning as nufft_custom_fft_f30_api.h:6-34 (fixed

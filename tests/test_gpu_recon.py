@@ -120,7 +120,13 @@ def test_cg_matches_fp64_oracle(precision):
     emu, emu_hist, _ = cg_native(model, y, 10, scale)
     assert nudft.complex_rel_l2(got, emu) <= 1e-3
     hist = res.residuals.cpu().numpy()
-    np.testing.assert_allclose(hist, np.asarray(emu_hist) , rtol=1e-3)
+    # late residuals are ~2e-3 of the first and carry the amplified rounding
+    # differences of every earlier step (BF16 tiles: 0.6% and 1.4% at the last
+    # step on two builds with identical operators), so the history is held
+    # relative to its entry and absolutely to 1e-4 of the initial residual
+    emu_hist = np.asarray(emu_hist)
+    np.testing.assert_allclose(hist, emu_hist, rtol=1e-2 if precision == "splitbf16" else 1e-3,
+                               atol=1e-4 * emu_hist[0])
 
 
 def test_graph_replay_is_bitwise_equal_to_eager():

@@ -16,6 +16,7 @@ TSN_OK, TSN_ERR_RUNTIME, TSN_ERR_INCOMPATIBLE, TSN_ERR_CG, TSN_ERR_ARGUMENT = 0,
 PREC_SPLIT16, PREC_DENSE16, PREC_FP32, PREC_HI16, PREC_SPLITBF16 = 0, 1, 2, 3, 4
 VALUES_FP16, VALUES_BF16 = 0, 1
 OUT_COIL_MAJOR, OUT_ROW_MAJOR = 0, 1
+DC_MATERIALIZED, DC_FUSED = 1, 2
 
 
 class PackHeader(C.Structure):
@@ -29,6 +30,10 @@ class PackHeader(C.Structure):
 class PlanConfig(C.Structure):
     _fields_ = [("image0", C.c_uint32), ("image1", C.c_uint32), ("grid0", C.c_uint32),
                 ("grid1", C.c_uint32), ("coils", C.c_uint32), ("precision", C.c_int)]
+
+
+class BatchProblem(C.Structure):
+    _fields_ = [("forward", C.c_void_p), ("adjoint", C.c_void_p), ("smaps", C.c_void_p), ("coils", C.c_uint32)]
 
 
 class TsnError(RuntimeError):
@@ -66,6 +71,7 @@ SIGNATURES = {
     "tsn_direction_bytes": (C.c_size_t, [P]),
     "tsn_direction_schedule": (C.c_int, [P, C.c_uint32]),
     "tsn_direction_unit_tiles": (C.c_uint32, [P]),
+    "tsn_direction_autotune": (C.c_int, [P, C.c_uint32, P, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
     "tsn_direction_free": (None, [P]),
     "tsn_interp": (C.c_int, [P, P, P, P, U32, C.c_int, C.c_int, C.c_float, C.c_int, P]),
     "tsn_interp_dense": (C.c_int, [P, P, P, P, U32, C.c_int, C.c_int, C.c_float, C.c_int, P]),
@@ -73,15 +79,25 @@ SIGNATURES = {
     "tsn_split_panel": (C.c_int, [P, U32, U32, C.c_float, P, P, P]),
     "tsn_split_panel_il": (C.c_int, [P, U32, U32, C.c_float, C.c_int, P, P]),
     "tsn_interp_il": (C.c_int, [P, P, P, U32, C.c_int, C.c_int, C.c_float, C.c_int, P]),
+    "tsn_interp_dc": (C.c_int, [P, P, P, C.c_int, P, U32, C.c_float, C.c_float, C.c_int, P, P, P]),
     "tsn_plan_create": (C.c_int, [C.POINTER(PlanConfig), P, P, P, P, PP]),
     "tsn_plan_free": (None, [P]),
     "tsn_plan_bytes": (C.c_size_t, [P]),
     "tsn_forward": (C.c_int, [P, P, P, P]),
     "tsn_adjoint": (C.c_int, [P, P, P, P]),
     "tsn_normal": (C.c_int, [P, P, P, C.c_float, C.c_float, P]),
+    "tsn_data_consistency": (C.c_int, [P, P, P, P, C.c_float, P, P]),
     "tsn_cg": (C.c_int, [P, P, U32, C.c_float, C.c_float, P, P, P, P]),
     "tsn_cg_graph_capture": (C.c_int, [P, P, U32, C.c_float, C.c_float, P, P, P, P]),
     "tsn_cg_graph_launch": (C.c_int, [P, P]),
+    "tsn_batch_create": (C.c_int, [C.POINTER(PlanConfig), C.POINTER(BatchProblem), U32, P, PP]),
+    "tsn_batch_free": (None, [P]),
+    "tsn_batch_bytes": (C.c_size_t, [P]),
+    "tsn_batch_cg": (C.c_int, [P, C.POINTER(C.c_void_p), U32, C.POINTER(C.c_float), C.c_float,
+                               C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), P, P]),
+    "tsn_batch_cg_graph_capture": (C.c_int, [P, C.POINTER(C.c_void_p), U32, C.POINTER(C.c_float), C.c_float,
+                                             C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), P, P]),
+    "tsn_batch_cg_graph_launch": (C.c_int, [P, P]),
     "jkf_initialize_custom_fft_twiddles": (C.c_int, []),
     "jkf_launch_custom_fft_f30_forward": (None, [P, P, P, P, P]),
     "jkf_launch_custom_fft_f30_forward_split2": (None, [P, P, P, P, P, P]),

@@ -27,7 +27,7 @@ LIBRARY = PACKAGE / "libtsn.so"
 CLI = PACKAGE / "tsn_cg"
 ARCH = os.environ.get("TSN_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["pack_io.cpp", "packer.cpp", "interp.hip", "fft.hip", "plan.hip", "fft_compat.hip"]
+SOURCES = ["pack_io.cpp", "packer.cpp", "interp.hip", "fft.hip", "plan.hip", "batch.hip", "fft_compat.hip"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", f"-I{INCLUDE}",
           "-Wall", "-Wno-unused-function"]
 DEVICE = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -1,0 +1,460 @@
+// Batched CG-SENSE: many independent reconstructions of one image / grid
+// shape (slices with uneven spoke and coil counts, or frames sharing one
+// packed trajectory) advance in lockstep, each step one launch for all of
+// them: the FFT passes put the problem in blockIdx.z, the interpolation
+// merges every problem's work units into one launch (units of a shared pack
+// run unit-major so its tiles stream from HBM once), and the CG vector
+// kernels put the problem in blockIdx.y.  Scalars stay per problem (float64).
+//
+// Semantics per problem are exactly tsn_cg's (same kernels, same reduction
+// order), so a batched solve reproduces the single-plan solve bit for bit.
+#include <cmath>
+#include <vector>
+
+#include "cg_kernels.h"
+
+namespace tsn {
+int launch_split_panel_il(const float2* in, uint32_t rows, uint32_t coils, float scale, uint32_t* out,
+                          hipStream_t stream, int values);
+}  // namespace tsn
+
+using namespace tsn;
+
+namespace {
+
+// measurement z (coil-major [coils][m_z], raw) * scale -> interleaved split
+// sample panel of problem z (blocks of [m_z][16]).
+template <bool BF16>
+__global__ __launch_bounds__(256) void batch_split_kernel(const float2* const* __restrict__ meas,
+                                                          const uint32_t* __restrict__ coils_of,
+                                                          const uint32_t* __restrict__ m_of, uint32_t blocks_max,
+                                                          float scale, uint32_t* __restrict__ panels,
+                                                          uint64_t panel_stride) {
+  const uint32_t z = blockIdx.y;
+  const uint32_t m = m_of[z], coils = coils_of[z];
+  const uint32_t blocks = (coils + kCoilsPerBlock - 1) / kCoilsPerBlock;
+  const uint64_t total = static_cast<uint64_t>(blocks) * m * kN;
+  uint32_t* out = panels + z * panel_stride;
+  for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const int n = static_cast<int>(g & 15);
+    const uint64_t br = g >> 4;
+    const uint32_t row = static_cast<uint32_t>(br % m), b = static_cast<uint32_t>(br / m);
+    const uint32_t coil = b * kCoilsPerBlock + (n & 7);
+    float v = 0.f;
+    if (coil < coils) {
+      const float2 x = meas[z][static_cast<size_t>(coil) * m + row];
+      v = (n < 8 ? x.x : x.y) * scale;
+    }
+    uint32_t w;
+    if constexpr (BF16) {
+      w = split_bf16(v);
+    } else {
+      const _Float16 h = static_cast<_Float16>(v);
+      const _Float16 r = static_cast<_Float16>(v - static_cast<float>(h));
+      w = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
+          (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r)) << 16);
+    }
+    out[g] = w;
+  }
+  (void)blocks_max;
+}
+
+// x_z (column-major [n1][n0]) -> x_out[z] ([n0][n1]); history sqrt(rho) and status out.
+__global__ __launch_bounds__(256) void batch_out_kernel(const float2* __restrict__ x, uint64_t vstride,
+                                                        uint32_t n0, uint32_t n1, float2* const* __restrict__ xout,
+                                                        const double* __restrict__ hist, uint32_t hstride,
+                                                        uint32_t iterations, double* const* __restrict__ hist_out,
+                                                        const int* __restrict__ status, int* __restrict__ status_out) {
+  const uint32_t z = blockIdx.y;
+  const uint64_t px = static_cast<uint64_t>(n0) * n1;
+  for (uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < px;
+       e += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t i1 = static_cast<uint32_t>(e % n1), i0 = static_cast<uint32_t>(e / n1);
+    xout[z][e] = x[z * vstride + static_cast<uint64_t>(i1) * n0 + i0];
+  }
+  if (blockIdx.x == 0) {
+    for (uint32_t i = threadIdx.x; i <= iterations; i += blockDim.x) hist_out[z][i] = hist[z * hstride + i];
+    if (threadIdx.x == 0) status_out[z] = status[z];
+  }
+}
+
+}  // namespace
+
+struct tsn_batch {
+  tsn_plan_config cfg{};
+  int device = 0;
+  uint32_t problems = 0, cmax = 0, bmax = 0, mmax = 0, pixels = 0, vec_blocks = 0, grid = 0;
+  std::vector<uint32_t> coils, m;
+  std::vector<const tsn_direction*> fwd, adj;
+  InterpBatch ib_fwd, ib_adj;
+  std::vector<void*> allocations;
+  size_t bytes = 0;
+  float2 *tw0 = nullptr, *tw1 = nullptr, *smaps_t = nullptr, *t_fwd = nullptr, *t_inv = nullptr, *grid_buf = nullptr;
+  float* apod_t = nullptr;
+  uint32_t *pan_il = nullptr, *smp_il = nullptr, *coils_dev = nullptr, *m_dev = nullptr;
+  float2 *x = nullptr, *r = nullptr, *p = nullptr, *p2 = nullptr, *ap = nullptr, *b = nullptr;
+  double *partial = nullptr, *rr_partial = nullptr, *hist = nullptr;
+  uint32_t hist_cap = 0;
+  int* status = nullptr;
+  // per-call tables (device): measurement / output pointers and inverse-pass scales
+  const float2** meas_tab = nullptr;
+  float2** xout_tab = nullptr;
+  double** hist_tab = nullptr;
+  float2 *scale_rhs = nullptr, *scale_normal = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+template <typename T>
+int batch_alloc(tsn_batch* bt, T** ptr, size_t count) {
+  *ptr = nullptr;
+  if (count == 0) return TSN_OK;
+  TSN_HIP(hipMalloc(reinterpret_cast<void**>(ptr), count * sizeof(T)));
+  bt->allocations.push_back(*ptr);
+  bt->bytes += count * sizeof(T);
+  return TSN_OK;
+}
+
+void batch_destroy(tsn_batch* bt) {
+  if (!bt) return;
+  if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
+  if (bt->graph) (void)hipGraphDestroy(bt->graph);
+  interp_batch_free(&bt->ib_fwd);
+  interp_batch_free(&bt->ib_adj);
+  for (void* q : bt->allocations) (void)hipFree(q);
+  delete bt;
+}
+
+int interp_kind_of(int precision) {
+  switch (precision) {
+    case TSN_PREC_DENSE16: return 1;
+    case TSN_PREC_HI16: return 2;
+    case TSN_PREC_SPLITBF16: return 3;
+    default: return 0;
+  }
+}
+
+uint64_t panel_stride(const tsn_batch* bt) { return static_cast<uint64_t>(bt->bmax) * bt->grid * kN; }
+uint64_t sample_stride(const tsn_batch* bt) { return static_cast<uint64_t>(bt->bmax) * bt->mmax * kN; }
+
+FftArgs batch_fft_args(const tsn_batch* bt) {
+  FftArgs a{};
+  a.n0 = bt->cfg.image0;
+  a.n1 = bt->cfg.image1;
+  a.k0 = bt->cfg.grid0;
+  a.k1 = bt->cfg.grid1;
+  a.coils = bt->cmax;
+  a.cstride = bt->cmax;
+  a.coils_of = bt->coils_dev;
+  a.tw0 = bt->tw0;
+  a.tw1 = bt->tw1;
+  a.smaps_t = bt->smaps_t;
+  a.apod_t = bt->apod_t;
+  a.t_fwd = bt->t_fwd;
+  a.panel_il = bt->pan_il;
+  a.grid = bt->grid_buf;
+  a.t_inv = bt->t_inv;
+  a.image_stride = bt->pixels;
+  a.smaps_stride = static_cast<uint64_t>(bt->pixels) * bt->cmax;
+  a.t_stride = static_cast<uint64_t>(bt->cfg.grid0) * bt->cfg.image1 * bt->cmax;
+  a.grid_stride = static_cast<uint64_t>(bt->grid) * bt->cmax;
+  a.panel_stride = panel_stride(bt);
+  a.partial_stride = bt->cfg.image1;
+  a.rr_stride = bt->vec_blocks;
+  a.hist_stride = bt->hist_cap;
+  return a;
+}
+
+int out_kind_of(int precision) { return precision == TSN_PREC_SPLITBF16 ? 2 : 0; }
+
+int enqueue_batch_cg(tsn_batch* bt, uint32_t iterations, hipStream_t s) {
+  const uint32_t S = bt->problems;
+  const int kind = interp_kind_of(bt->cfg.precision);
+  const float norm = static_cast<float>(1.0 / std::sqrt(static_cast<double>(bt->cfg.grid0) * bt->cfg.grid1));
+  const uint32_t n = bt->pixels, vb = bt->vec_blocks;
+  const uint64_t vs = bt->pixels;
+  // b = A^H (y / sqrt(K0K1)) / s^2 per problem
+  const uint64_t total = static_cast<uint64_t>(bt->bmax) * bt->mmax * kN;
+  const dim3 split_grid(std::min<uint64_t>(div_up(total, 256), 4096), S);
+  if (bt->cfg.precision == TSN_PREC_SPLITBF16)
+    hipLaunchKernelGGL(batch_split_kernel<true>, split_grid, dim3(256), 0, s, bt->meas_tab, bt->coils_dev,
+                       bt->m_dev, bt->bmax, norm, bt->smp_il, sample_stride(bt));
+  else
+    hipLaunchKernelGGL(batch_split_kernel<false>, split_grid, dim3(256), 0, s, bt->meas_tab, bt->coils_dev,
+                       bt->m_dev, bt->bmax, norm, bt->smp_il, sample_stride(bt));
+  TSN_CHECK(launch_interp_batch(bt->ib_adj, kind, 1, true, 1.0f, s));
+  {
+    FftArgs a = batch_fft_args(bt);
+    a.image_out = bt->b;
+    a.scale_of = bt->scale_rhs;
+    TSN_CHECK(fft_inverse(a, s, S));
+  }
+  hipLaunchKernelGGL(cg_init_kernel, dim3(vb, S), dim3(kVecThreads), 0, s, bt->b, bt->x, bt->r, bt->p, n,
+                     bt->rr_partial, vs, vb);
+  hipLaunchKernelGGL(cg_rho_kernel, dim3(1, S), dim3(kVecThreads), 0, s, bt->rr_partial, vb, bt->hist, 0u, 0,
+                     bt->status, vb, bt->hist_cap);
+  float2* p_cur = bt->p;
+  float2* p_other = bt->p2;
+  for (uint32_t it = 0; it < iterations; ++it) {
+    FftArgs a = batch_fft_args(bt);
+    if (it == 0) {
+      a.image_t = p_cur;
+    } else {
+      a.cg_r = bt->r;
+      a.cg_p_old = p_cur;
+      a.cg_p_new = p_other;
+      a.rr_partial = bt->rr_partial;
+      a.rr_count = vb;
+      a.rho_hist = bt->hist;
+      a.iter = it;
+      std::swap(p_cur, p_other);
+    }
+    TSN_CHECK(fft_forward(a, out_kind_of(bt->cfg.precision), s, S));
+    TSN_CHECK(launch_interp_batch(bt->ib_fwd, kind, 3, false, norm, s));
+    TSN_CHECK(launch_interp_batch(bt->ib_adj, kind, 1, true, 1.0f, s));
+    FftArgs inv = batch_fft_args(bt);
+    inv.image_out = bt->ap;
+    inv.reg_in = p_cur;
+    inv.scale_of = bt->scale_normal;
+    inv.dot_partial = bt->partial;
+    TSN_CHECK(fft_inverse(inv, s, S));
+    hipLaunchKernelGGL(cg_update_xr_kernel, dim3(vb, S), dim3(kVecThreads), 0, s, bt->x, bt->r, p_cur, bt->ap,
+                       bt->partial, bt->cfg.image1, bt->hist, it, bt->status, n, bt->rr_partial, vs,
+                       bt->cfg.image1, bt->hist_cap, vb);
+  }
+  hipLaunchKernelGGL(cg_rho_kernel, dim3(1, S), dim3(kVecThreads), 0, s, bt->rr_partial, vb, bt->hist, iterations,
+                     1, bt->status, vb, bt->hist_cap);
+  TSN_HIP(hipGetLastError());
+  return TSN_OK;
+}
+
+int enqueue_batch_out(tsn_batch* bt, uint32_t iterations, int* status_out, hipStream_t s) {
+  hipLaunchKernelGGL(batch_out_kernel, dim3(std::min<uint32_t>(div_up(bt->pixels, 256), 1024), bt->problems),
+                     dim3(256), 0, s, bt->x, static_cast<uint64_t>(bt->pixels), bt->cfg.image0, bt->cfg.image1,
+                     bt->xout_tab, bt->hist, bt->hist_cap, iterations, bt->hist_tab, bt->status, status_out);
+  TSN_HIP(hipGetLastError());
+  return TSN_OK;
+}
+
+// Upload the per-call tables (synchronously: the caller's stream is drained
+// first, so no queued launch still reads the previous tables).
+int stage_call(tsn_batch* bt, const void* const* meas, uint32_t iterations, const float* scales, float lambda,
+               void* const* x_out, void* const* residuals, hipStream_t s) {
+  const uint32_t S = bt->problems;
+  TSN_HIP(hipSetDevice(bt->device));
+  TSN_HIP(hipStreamSynchronize(s));
+  if (iterations + 1 > bt->hist_cap) {
+    if (bt->graph) return fail(TSN_ERR_ARGUMENT, "iterations exceed the captured batch's history capacity");
+    double* h = nullptr;
+    TSN_HIP(hipMalloc(reinterpret_cast<void**>(&h), static_cast<size_t>(S) * (iterations + 1) * sizeof(double)));
+    bt->allocations.push_back(h);
+    bt->hist = h;
+    bt->hist_cap = iterations + 1;
+  }
+  const double kk = static_cast<double>(bt->cfg.grid0) * bt->cfg.grid1;
+  const float norm = static_cast<float>(1.0 / std::sqrt(kk));
+  std::vector<float2> rhs(S), nrm(S);
+  for (uint32_t z = 0; z < S; ++z) {
+    const double sq = static_cast<double>(scales[z]) * scales[z];
+    const float inv = static_cast<float>(1.0 / sq);
+    rhs[z] = make_float2(inv * norm, 0.0f);
+    nrm[z] = make_float2(inv * norm, static_cast<float>(lambda / kk / sq));
+  }
+  TSN_HIP(hipMemcpy(bt->meas_tab, meas, S * sizeof(void*), hipMemcpyHostToDevice));
+  TSN_HIP(hipMemcpy(bt->xout_tab, x_out, S * sizeof(void*), hipMemcpyHostToDevice));
+  TSN_HIP(hipMemcpy(bt->hist_tab, residuals, S * sizeof(void*), hipMemcpyHostToDevice));
+  TSN_HIP(hipMemcpy(bt->scale_rhs, rhs.data(), S * sizeof(float2), hipMemcpyHostToDevice));
+  TSN_HIP(hipMemcpy(bt->scale_normal, nrm.data(), S * sizeof(float2), hipMemcpyHostToDevice));
+  return TSN_OK;
+}
+
+int check_batch_call(tsn_batch* bt, const void* const* meas, uint32_t iterations, const float* scales,
+                     float lambda, void* const* x_out, void* const* residuals, const void* status) {
+  if (!bt || !meas || !scales || !x_out || !residuals || !status) return fail(TSN_ERR_ARGUMENT, "null argument");
+  if (iterations == 0) return fail(TSN_ERR_ARGUMENT, "cg iterations must be positive");
+  if (!(lambda >= 0.0f) || !std::isfinite(lambda)) return fail(TSN_ERR_ARGUMENT, "lambda must be >= 0");
+  for (uint32_t z = 0; z < bt->problems; ++z) {
+    if (!meas[z] || !x_out[z] || !residuals[z]) return fail(TSN_ERR_ARGUMENT, "null pointer for problem %u", z);
+    if (!(scales[z] > 0.0f) || !std::isfinite(scales[z]))
+      return fail(TSN_ERR_ARGUMENT, "operator scale of problem %u must be positive and finite", z);
+  }
+  return TSN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tsn_batch_create(const tsn_plan_config* cfg, const tsn_batch_problem* problems, uint32_t count,
+                     const void* apodization, tsn_batch** out) {
+  if (!cfg || !problems || !apodization || !out || count == 0) return fail(TSN_ERR_ARGUMENT, "bad arguments");
+  *out = nullptr;
+  const tsn_plan_config c = *cfg;
+  if (c.grid0 != 2 * c.image0 || c.grid1 != 2 * c.image1 || !fft_length_supported(c.grid0) ||
+      !fft_length_supported(c.grid1))
+    return fail(TSN_ERR_ARGUMENT, "grid must be 2x image with grid sides in {64, 128, 256, 512, 768}");
+  if (c.precision == TSN_PREC_FP32 || c.precision < TSN_PREC_SPLIT16 || c.precision > TSN_PREC_SPLITBF16)
+    return fail(TSN_ERR_ARGUMENT, "batches run the packed variants (split16, dense16, hi16, splitbf16)");
+  const int want = c.precision == TSN_PREC_SPLITBF16 ? TSN_VALUES_BF16 : TSN_VALUES_FP16;
+  const uint32_t grid = c.grid0 * c.grid1;
+  auto* bt = new tsn_batch();
+  bt->cfg = c;
+  (void)hipGetDevice(&bt->device);
+  bt->problems = count;
+  bt->grid = grid;
+  bt->pixels = c.image0 * c.image1;
+  bt->vec_blocks = div_up(bt->pixels, kVecThreads);
+  for (uint32_t z = 0; z < count; ++z) {
+    const tsn_batch_problem& q = problems[z];
+    int rc = TSN_OK;
+    if (!q.forward || !q.adjoint || !q.smaps) rc = fail(TSN_ERR_ARGUMENT, "problem %u: null argument", z);
+    else if (q.coils == 0 || q.coils > static_cast<uint32_t>(kMaxCoils))
+      rc = fail(TSN_ERR_ARGUMENT, "problem %u: coils must be in 1..16", z);
+    else if (q.forward->csr || q.adjoint->csr || q.forward->values != want || q.adjoint->values != want)
+      rc = fail(TSN_ERR_ARGUMENT, "problem %u: packs do not match precision %d", z, c.precision);
+    else if (q.forward->h.n_cols != grid || q.adjoint->h.output_size != grid ||
+             q.forward->h.output_size != q.adjoint->h.n_cols)
+      rc = fail(TSN_ERR_INCOMPATIBLE, "problem %u: unexpected operator shape", z);
+    else if (c.precision == TSN_PREC_DENSE16 && (!q.forward->tile_dense || !q.adjoint->tile_dense))
+      rc = fail(TSN_ERR_ARGUMENT, "problem %u: dense control needs dense tiles", z);
+    if (rc != TSN_OK) {
+      batch_destroy(bt);
+      return rc;
+    }
+    bt->coils.push_back(q.coils);
+    bt->m.push_back(q.forward->h.output_size);
+    bt->fwd.push_back(q.forward);
+    bt->adj.push_back(q.adjoint);
+    bt->cmax = std::max(bt->cmax, q.coils);
+    bt->mmax = std::max(bt->mmax, q.forward->h.output_size);
+  }
+  bt->bmax = div_up(bt->cmax, kCoilsPerBlock);
+  const size_t px = bt->pixels, S = count;
+  int rc = TSN_OK;
+  auto A = [&](auto** ptr, size_t n) {
+    if (rc == TSN_OK) rc = batch_alloc(bt, ptr, n);
+  };
+  A(&bt->tw0, c.grid0);
+  A(&bt->tw1, c.grid1);
+  A(&bt->smaps_t, S * px * bt->cmax);
+  A(&bt->apod_t, px);
+  A(&bt->t_fwd, S * c.grid0 * c.image1 * bt->cmax);
+  A(&bt->t_inv, S * c.grid0 * c.image1 * bt->cmax);
+  A(&bt->grid_buf, S * grid * bt->cmax);
+  A(&bt->pan_il, S * panel_stride(bt));
+  A(&bt->smp_il, S * sample_stride(bt));
+  for (float2** v : {&bt->x, &bt->r, &bt->p, &bt->p2, &bt->ap, &bt->b}) A(v, S * px);
+  A(&bt->partial, S * c.image1);
+  A(&bt->rr_partial, S * bt->vec_blocks);
+  A(&bt->status, S);
+  A(&bt->coils_dev, S);
+  A(&bt->m_dev, S);
+  A(&bt->meas_tab, S);
+  A(&bt->xout_tab, S);
+  A(&bt->hist_tab, S);
+  A(&bt->scale_rhs, S);
+  A(&bt->scale_normal, S);
+  if (rc == TSN_OK) {
+    const auto t0 = twiddles(c.grid0), t1 = twiddles(c.grid1);
+    if (hipMemcpy(bt->tw0, t0.data(), t0.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(bt->tw1, t1.data(), t1.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(bt->coils_dev, bt->coils.data(), S * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(bt->m_dev, bt->m.data(), S * 4, hipMemcpyHostToDevice) != hipSuccess)
+      rc = fail(TSN_ERR_RUNTIME, "batch setup upload failed");
+  }
+  if (rc == TSN_OK) {
+    rc = zero_async(bt->grid_buf, S * grid * bt->cmax * sizeof(float2), 0);
+    if (rc == TSN_OK) rc = zero_async(bt->smaps_t, S * px * bt->cmax * sizeof(float2), 0);
+  }
+  for (uint32_t z = 0; rc == TSN_OK && z < count; ++z) {
+    const uint64_t ns = static_cast<uint64_t>(bt->coils[z]) * px;
+    hipLaunchKernelGGL(smaps_transpose_kernel, dim3(div_up(ns, 256)), dim3(256), 0, 0,
+                       static_cast<const float2*>(problems[z].smaps), bt->coils[z], c.image0, c.image1, bt->cmax,
+                       bt->smaps_t + z * px * bt->cmax);
+  }
+  if (rc == TSN_OK) {
+    hipLaunchKernelGGL(apod_transpose_kernel, dim3(div_up(px, 256)), dim3(256), 0, 0,
+                       static_cast<const float*>(apodization), c.image0, c.image1, bt->apod_t);
+    const hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) rc = fail(TSN_ERR_RUNTIME, "batch setup: %s", hipGetErrorString(e));
+  }
+  // interpolation parts: one per coil block of each problem
+  if (rc == TSN_OK) {
+    std::vector<InterpBatchPart> fw, ad;
+    for (uint32_t z = 0; z < count; ++z) {
+      const uint32_t blocks = div_up(bt->coils[z], kCoilsPerBlock);
+      for (uint32_t k = 0; k < blocks; ++k) {
+        uint32_t* pan = bt->pan_il + z * panel_stride(bt) + static_cast<uint64_t>(k) * grid * kN;
+        uint32_t* smp = bt->smp_il + z * sample_stride(bt) + static_cast<uint64_t>(k) * bt->m[z] * kN;
+        float* grid_out = reinterpret_cast<float*>(bt->grid_buf + z * grid * bt->cmax);
+        fw.push_back({bt->fwd[z], pan, nullptr, smp, bt->cmax, k * kCoilsPerBlock});
+        ad.push_back({bt->adj[z], smp, grid_out, nullptr, bt->cmax, k * kCoilsPerBlock});
+      }
+    }
+    rc = interp_batch_build(fw, &bt->ib_fwd);
+    if (rc == TSN_OK) rc = interp_batch_build(ad, &bt->ib_adj);
+  }
+  if (rc != TSN_OK) {
+    batch_destroy(bt);
+    return rc;
+  }
+  *out = bt;
+  return TSN_OK;
+}
+
+void tsn_batch_free(tsn_batch* bt) { batch_destroy(bt); }
+
+size_t tsn_batch_bytes(const tsn_batch* bt) { return bt ? bt->bytes : 0; }
+
+int tsn_batch_cg(tsn_batch* bt, const void* const* measurements, uint32_t iterations, const float* operator_scales,
+                 float lambda, void* const* x_out, void* const* residuals_out, void* status_out, void* stream) {
+  TSN_CHECK(check_batch_call(bt, measurements, iterations, operator_scales, lambda, x_out, residuals_out,
+                             status_out));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (bt->graph) return fail(TSN_ERR_ARGUMENT, "this batch holds a captured graph; launch it or free the batch");
+  TSN_CHECK(stage_call(bt, measurements, iterations, operator_scales, lambda, x_out, residuals_out, s));
+  TSN_CHECK(enqueue_batch_cg(bt, iterations, s));
+  return enqueue_batch_out(bt, iterations, static_cast<int*>(status_out), s);
+}
+
+int tsn_batch_cg_graph_capture(tsn_batch* bt, const void* const* measurements, uint32_t iterations,
+                               const float* operator_scales, float lambda, void* const* x_out,
+                               void* const* residuals_out, void* status_out, void* stream) {
+  TSN_CHECK(check_batch_call(bt, measurements, iterations, operator_scales, lambda, x_out, residuals_out,
+                             status_out));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!s) return fail(TSN_ERR_ARGUMENT, "graph capture needs a non-null stream");
+  if (bt->exec) {
+    (void)hipGraphExecDestroy(bt->exec);
+    bt->exec = nullptr;
+  }
+  if (bt->graph) {
+    (void)hipGraphDestroy(bt->graph);
+    bt->graph = nullptr;
+  }
+  TSN_CHECK(stage_call(bt, measurements, iterations, operator_scales, lambda, x_out, residuals_out, s));
+  TSN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeGlobal));
+  int rc = enqueue_batch_cg(bt, iterations, s);
+  if (rc == TSN_OK) rc = enqueue_batch_out(bt, iterations, static_cast<int*>(status_out), s);
+  hipGraph_t g = nullptr;
+  const hipError_t end = hipStreamEndCapture(s, &g);
+  if (rc != TSN_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (end != hipSuccess) return fail(TSN_ERR_RUNTIME, "hipStreamEndCapture: %s", hipGetErrorString(end));
+  bt->graph = g;
+  TSN_HIP(hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0));
+  return TSN_OK;
+}
+
+int tsn_batch_cg_graph_launch(tsn_batch* bt, void* stream) {
+  if (!bt || !bt->exec) return fail(TSN_ERR_ARGUMENT, "no captured graph");
+  TSN_HIP(hipGraphLaunch(bt->exec, static_cast<hipStream_t>(stream)));
+  return TSN_OK;
+}
+
+}  // extern "C"

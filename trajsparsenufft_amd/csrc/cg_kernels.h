@@ -1,0 +1,204 @@
+// Vector kernels shared by plans and batches: CG updates with float64
+// scalars (blockIdx.y = problem of a batched launch, pointers advanced by the
+// given strides; a single plan passes strides 0), transposes, fills.
+#pragma once
+
+#include <cmath>
+#include <vector>
+
+#include "fft_common.h"
+
+namespace {
+
+using tsn::div_up;
+using tsn::fail;
+
+constexpr int kVecThreads = 256;
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (static_cast<int>(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+// Fixed-order sum of partials: every block that calls it gets the same value.
+__device__ __forceinline__ double sum_partials(const double* __restrict__ partial, uint32_t count, double* red) {
+  double s = 0.0;
+  for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) s += partial[i];
+  return block_sum(s, red);
+}
+
+// x = 0, r = p = b, partial <r, r>
+__global__ __launch_bounds__(kVecThreads) void cg_init_kernel(const float2* __restrict__ b,
+                                                              float2* __restrict__ x, float2* __restrict__ r,
+                                                              float2* __restrict__ p, uint32_t n,
+                                                              double* __restrict__ partial, uint64_t vstride,
+                                                              uint32_t pstride) {
+  __shared__ double red[kVecThreads];
+  const uint64_t z = blockIdx.y;  // problem of a batched launch
+  b += z * vstride;
+  x += z * vstride;
+  r += z * vstride;
+  p += z * vstride;
+  partial += z * pstride;
+  double s = 0.0;
+  const uint32_t e = blockIdx.x * kVecThreads + threadIdx.x;
+  if (e < n) {
+    const float2 v = b[e];
+    x[e] = make_float2(0.f, 0.f);
+    r[e] = v;
+    p[e] = v;
+    s = static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y;
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// alpha = rho_k / Re<p, Ap> (0 and status 0 on non-positive or non-finite
+// curvature), x += alpha p, r -= alpha Ap, partial <r, r>.  rho_k is
+// rho_hist[iter]; every block reduces the same <p, Ap> partials itself.
+__global__ __launch_bounds__(kVecThreads) void cg_update_xr_kernel(
+    float2* __restrict__ x, float2* __restrict__ r, const float2* __restrict__ p,
+    const float2* __restrict__ ap, const double* __restrict__ dot_partial, uint32_t dot_count,
+    const double* __restrict__ rho_hist, uint32_t iter, int* __restrict__ status, uint32_t n,
+    double* __restrict__ rr_partial, uint64_t vstride, uint32_t dstride, uint32_t hstride, uint32_t rstride) {
+  __shared__ double red[kVecThreads];
+  const uint64_t z = blockIdx.y;  // problem of a batched launch
+  x += z * vstride;
+  r += z * vstride;
+  p += z * vstride;
+  ap += z * vstride;
+  dot_partial += z * dstride;
+  rho_hist += z * hstride;
+  status += z;
+  rr_partial += z * rstride;
+  const double den = sum_partials(dot_partial, dot_count, red);
+  __syncthreads();
+  double alpha = 0.0;
+  if (isfinite(den) && den > 0.0) alpha = rho_hist[iter] / den;
+  else if (blockIdx.x == 0 && threadIdx.x == 0) *status = 0;
+  double s = 0.0;
+  const uint32_t e = blockIdx.x * kVecThreads + threadIdx.x;
+  if (e < n) {
+    const float2 pv = p[e], av = ap[e];
+    float2 xv = x[e], rv = r[e];
+    xv.x = static_cast<float>(xv.x + alpha * pv.x);
+    xv.y = static_cast<float>(xv.y + alpha * pv.y);
+    rv.x = static_cast<float>(rv.x - alpha * av.x);
+    rv.y = static_cast<float>(rv.y - alpha * av.y);
+    x[e] = xv;
+    r[e] = rv;
+    s = static_cast<double>(rv.x) * rv.x + static_cast<double>(rv.y) * rv.y;
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) rr_partial[blockIdx.x] = s;
+}
+
+// rho_hist[index] = sum of partials (status = 1 when index == 0).  With
+// finish, rho_hist[0 .. index] then become the residual history sqrt(rho).
+__global__ __launch_bounds__(kVecThreads) void cg_rho_kernel(const double* __restrict__ partial,
+                                                             uint32_t count, double* __restrict__ rho_hist,
+                                                             uint32_t index, int finish,
+                                                             int* __restrict__ status, uint32_t pstride,
+                                                             uint32_t hstride) {
+  __shared__ double red[kVecThreads];
+  const uint64_t z = blockIdx.y;  // problem of a batched launch
+  partial += z * pstride;
+  rho_hist += z * hstride;
+  status += z;
+  const double s = sum_partials(partial, count, red);
+  if (threadIdx.x == 0) {
+    rho_hist[index] = s;
+    if (index == 0) *status = 1;
+  }
+  if (!finish) return;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i <= index; i += blockDim.x) rho_hist[i] = sqrt(fmax(rho_hist[i], 0.0));
+}
+
+// [rows][cols] -> [cols][rows] complex, 32x32 tiles, optional scale.
+__global__ __launch_bounds__(256) void transpose_kernel(const float2* __restrict__ in, uint32_t rows,
+                                                        uint32_t cols, float scale,
+                                                        float2* __restrict__ out) {
+  __shared__ float2 tile[32][33];
+  const uint32_t bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  for (uint32_t i = threadIdx.y; i < 32; i += 8) {
+    const uint32_t r = by + i, c = bx + threadIdx.x;
+    if (r < rows && c < cols) tile[i][threadIdx.x] = in[static_cast<size_t>(r) * cols + c];
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.y; i < 32; i += 8) {
+    const uint32_t c = bx + i, r = by + threadIdx.x;
+    if (r < rows && c < cols) {
+      const float2 v = tile[threadIdx.x][i];
+      out[static_cast<size_t>(c) * rows + r] = make_float2(v.x * scale, v.y * scale);
+    }
+  }
+}
+
+// [coils][n0][n1] -> [n1][cstride][n0]
+__global__ __launch_bounds__(256) void smaps_transpose_kernel(const float2* __restrict__ in, uint32_t coils,
+                                                              uint32_t n0, uint32_t n1, uint32_t cstride,
+                                                              float2* __restrict__ out) {
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t total = static_cast<uint64_t>(coils) * n0 * n1;
+  if (e >= total) return;
+  const uint32_t x1 = static_cast<uint32_t>(e % n1);
+  const uint32_t x0 = static_cast<uint32_t>((e / n1) % n0);
+  const uint32_t c = static_cast<uint32_t>(e / (static_cast<uint64_t>(n1) * n0));
+  out[(static_cast<size_t>(x1) * cstride + c) * n0 + x0] = in[e];
+}
+
+__global__ __launch_bounds__(256) void apod_transpose_kernel(const float* __restrict__ in, uint32_t n0,
+                                                             uint32_t n1, float* __restrict__ out) {
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<uint64_t>(n0) * n1) return;
+  const uint32_t x1 = static_cast<uint32_t>(e % n1), x0 = static_cast<uint32_t>(e / n1);
+  out[static_cast<size_t>(x1) * n0 + x0] = in[e];
+}
+
+__global__ __launch_bounds__(256) void fill_zero_kernel(float4* __restrict__ p, uint64_t n4) {
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < n4) p[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// [coils][m] -> [m][coils] * scale (FP32 path sample layout)
+__global__ __launch_bounds__(256) void samples_to_rows_kernel(const float2* __restrict__ in, uint32_t coils,
+                                                              uint32_t m, float scale,
+                                                              float2* __restrict__ out) {
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<uint64_t>(coils) * m) return;
+  const uint32_t c = static_cast<uint32_t>(e % coils), row = static_cast<uint32_t>(e / coils);
+  const float2 v = in[static_cast<size_t>(c) * m + row];
+  out[e] = make_float2(v.x * scale, v.y * scale);
+}
+
+__global__ void copy_status_kernel(const int* __restrict__ src, int* __restrict__ dst) {
+  if (threadIdx.x == 0) *dst = *src;
+}
+
+// a fill kernel (not hipMemsetAsync) so the work can sit inside a captured graph
+inline int zero_async(void* p, size_t bytes, hipStream_t s) {
+  if (bytes % 16) return fail(TSN_ERR_ARGUMENT, "zero fill needs a multiple of 16 bytes");
+  const uint64_t n4 = bytes / 16;
+  if (n4 == 0) return TSN_OK;
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(div_up(n4, 256)), dim3(256), 0, s, static_cast<float4*>(p), n4);
+  TSN_HIP(hipGetLastError());
+  return TSN_OK;
+}
+
+// W_K^k = exp(-2 pi i k / K), k < K, computed in double
+inline std::vector<float2> twiddles(uint32_t k) {
+  std::vector<float2> t(k);
+  for (uint32_t i = 0; i < k; ++i) {
+    const double ang = -2.0 * M_PI * static_cast<double>(i) / static_cast<double>(k);
+    t[i] = make_float2(static_cast<float>(std::cos(ang)), static_cast<float>(std::sin(ang)));
+  }
+  return t;
+}
+
+}  // namespace

@@ -25,7 +25,9 @@ namespace tsn {
 // Forward, first axis: image column n1 (all n0 < N0) of x*S_c*apod for eight
 // coils -> T[k0][n1][c].  Image-space arrays are column-major ([n1][n0]).
 template <int R>
-__global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs a) {
+__global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
+  const FftArgs a = problem_view(args, blockIdx.z);
+  if (blockIdx.y * 8 >= a.coils) return;  // coil chunk beyond this problem (batched launches)
   extern __shared__ float2 lds[];
   __shared__ double red[512];
   constexpr int K = 64 * R;
@@ -33,7 +35,28 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs a) {
   const uint32_t n1 = blockIdx.x, chunk = blockIdx.y;
   const uint32_t c = chunk * 8 + w;
   const bool active = c < a.coils;
-  double beta = 0.0;
+  const WaveTwiddles<R> twp = load_twiddles<R>(a.tw0, lane);
+  // issue every data load first (image or r and p, maps, apodization); the
+  // direction update's beta reduction then overlaps their latency
+  float2 img[R], sm[R], pv[R];
+  float ap[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    img[j] = pv[j] = sm[j] = make_float2(0.f, 0.f);
+    ap[j] = 0.f;
+    const uint32_t n0 = lane + 64 * j;
+    if (active && n0 < a.n0) {
+      const size_t pix = static_cast<size_t>(n1) * a.n0 + n0;
+      if (a.cg_r) {
+        img[j] = a.cg_r[pix];
+        pv[j] = a.cg_p_old[pix];
+      } else {
+        img[j] = a.image_t[pix];
+      }
+      sm[j] = a.smaps_t[(static_cast<size_t>(n1) * a.cstride + c) * a.n0 + n0];
+      ap[j] = a.apod_t[pix];
+    }
+  }
   if (a.cg_r) {
     // every block reduces the same partials in the same order: same beta
     double s = 0.0;
@@ -45,39 +68,30 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs a) {
       __syncthreads();
     }
     const double rho_new = red[0];
-    beta = rho_new / fmax(a.rho_hist[a.iter - 1], 1e-30);
+    const double beta = rho_new / fmax(a.rho_hist[a.iter - 1], 1e-30);
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.rho_hist[a.iter] = rho_new;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const uint32_t n0 = lane + 64 * j;
+      img[j] = make_float2(static_cast<float>(img[j].x + beta * pv[j].x), static_cast<float>(img[j].y + beta * pv[j].y));
+      if (active && n0 < a.n0 && chunk == 0 && w == 0) a.cg_p_new[static_cast<size_t>(n1) * a.n0 + n0] = img[j];
+    }
   }
   float2 x[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) x[j] = make_float2(0.f, 0.f);
-#pragma unroll
   for (int j = 0; j < R; ++j) {
-    const uint32_t n0 = lane + 64 * j;
-    if (active && n0 < a.n0) {
-      const size_t pix = static_cast<size_t>(n1) * a.n0 + n0;
-      float2 img;
-      if (a.cg_r) {
-        const float2 rv = a.cg_r[pix], pv = a.cg_p_old[pix];
-        img = make_float2(static_cast<float>(rv.x + beta * pv.x), static_cast<float>(rv.y + beta * pv.y));
-        if (chunk == 0 && w == 0) a.cg_p_new[pix] = img;
-      } else {
-        img = a.image_t[pix];
-      }
-      const float2 v = cmul(img, a.smaps_t[(static_cast<size_t>(n1) * a.coils + c) * a.n0 + n0]);
-      const float s = a.apod_t[pix];
-      x[j] = make_float2(v.x * s, v.y * s);
-    }
+    const float2 v = cmul(img[j], sm[j]);
+    x[j] = make_float2(v.x * ap[j], v.y * ap[j]);
   }
   constexpr int TPL = (R * 8 + 63) / 64;
   float2 out[TPL][8];
-  wave_fft<R, false, true, false>(x, lds + w * wave_lds_elems<R>(), a.tw0, lane, out);
+  wave_fft<R, false, true, false>(x, lds + w * wave_lds_elems<R>(), a.tw0, twp, lane, out);
   __syncthreads();
   emit_exchange<R, false>(out, lds, lane, w);
   __syncthreads();
   const uint32_t cw = min(8u, a.coils - chunk * 8);
   for (uint32_t k = threadIdx.x; k < static_cast<uint32_t>(K); k += blockDim.x) {
-    float2* dst = a.t_fwd + (static_cast<size_t>(k) * a.n1 + n1) * a.coils + chunk * 8;
+    float2* dst = a.t_fwd + (static_cast<size_t>(k) * a.n1 + n1) * a.cstride + chunk * 8;
     if (cw == 8) {
       float4* d4 = reinterpret_cast<float4*>(dst);
 #pragma unroll
@@ -95,17 +109,17 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs a) {
 // hi/residual interleaved N16 panel blocks; OUT 1: float2 [row][coils];
 // OUT 2: BF16 hi/residual interleaved panel blocks.
 template <int R, int OUT>
-__global__ __launch_bounds__(512) void fwd_axis1_kernel(FftArgs a) {
+__global__ __launch_bounds__(512) void fwd_axis1_kernel(FftArgs args) {
+  const FftArgs a = problem_view(args, blockIdx.z);
+  if (blockIdx.y * 8 >= a.coils) return;  // coil chunk beyond this problem (batched launches)
+  const WaveTwiddles<R> twp = load_twiddles<R>(a.tw1, static_cast<int>(threadIdx.x & 63));
   extern __shared__ float2 lds[];
   constexpr int K = 64 * R;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t k0 = blockIdx.x, chunk = blockIdx.y;
   const uint32_t cw = min(8u, a.coils - chunk * 8);
-  for (uint32_t e = threadIdx.x; e < a.n1 * 8; e += blockDim.x) {
-    const uint32_t n1 = e >> 3, c = e & 7;
-    lds[xslot(n1, c)] = c < cw ? a.t_fwd[(static_cast<size_t>(k0) * a.n1 + n1) * a.coils + chunk * 8 + c]
-                               : make_float2(0.f, 0.f);
-  }
+  // grid row k0 of T: N1 = K/2 rows (the plan checks grid = 2 x image)
+  fill_exchange<K / 2>(a.t_fwd + static_cast<size_t>(k0) * a.n1 * a.cstride + chunk * 8, a.cstride, cw, lds);
   __syncthreads();
   float2 x[R];
 #pragma unroll
@@ -116,7 +130,7 @@ __global__ __launch_bounds__(512) void fwd_axis1_kernel(FftArgs a) {
   __syncthreads();
   constexpr int TPL = (R * 8 + 63) / 64;
   float2 out[TPL][8];
-  wave_fft<R, false, true, false>(x, lds + w * wave_lds_elems<R>(), a.tw1, lane, out);
+  wave_fft<R, false, true, false>(x, lds + w * wave_lds_elems<R>(), a.tw1, twp, lane, out);
   __syncthreads();
   emit_exchange<R, false>(out, lds, lane, w);
   __syncthreads();
@@ -145,7 +159,7 @@ __global__ __launch_bounds__(512) void fwd_axis1_kernel(FftArgs a) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) d4[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
     } else {
-      float2* dst = a.panel32 + r * a.coils + chunk * 8;
+      float2* dst = a.panel32 + r * a.cstride + chunk * 8;
       for (uint32_t i = 0; i < cw; ++i) dst[i] = lds[xslot(k1, i)];
     }
   }
@@ -154,17 +168,16 @@ __global__ __launch_bounds__(512) void fwd_axis1_kernel(FftArgs a) {
 // Inverse, second grid axis: grid row k0 ([k0*K1 + k1][coils]) -> IDFT along
 // k1, keep n1 < N1 -> T2[n1][k0][c].
 template <int R>
-__global__ __launch_bounds__(512) void inv_axis1_kernel(FftArgs a) {
+__global__ __launch_bounds__(512) void inv_axis1_kernel(FftArgs args) {
+  const FftArgs a = problem_view(args, blockIdx.z);
+  if (blockIdx.y * 8 >= a.coils) return;  // coil chunk beyond this problem (batched launches)
+  const WaveTwiddles<R> twp = load_twiddles<R>(a.tw1, static_cast<int>(threadIdx.x & 63));
   extern __shared__ float2 lds[];
   constexpr int K = 64 * R;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t k0 = blockIdx.x, chunk = blockIdx.y;
   const uint32_t cw = min(8u, a.coils - chunk * 8);
-  for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(K) * 8; e += blockDim.x) {
-    const uint32_t k1 = e >> 3, c = e & 7;
-    lds[xslot(k1, c)] = c < cw ? a.grid[(static_cast<size_t>(k0) * K + k1) * a.coils + chunk * 8 + c]
-                               : make_float2(0.f, 0.f);
-  }
+  fill_exchange<K>(a.grid + static_cast<size_t>(k0) * K * a.cstride + chunk * 8, a.cstride, cw, lds);
   __syncthreads();
   float2 x[R];
 #pragma unroll
@@ -172,12 +185,12 @@ __global__ __launch_bounds__(512) void inv_axis1_kernel(FftArgs a) {
   __syncthreads();
   constexpr int TPL = (R * 8 + 63) / 64;
   float2 out[TPL][8];
-  wave_fft<R, true, false, true>(x, lds + w * wave_lds_elems<R>(), a.tw1, lane, out);
+  wave_fft<R, true, false, true>(x, lds + w * wave_lds_elems<R>(), a.tw1, twp, lane, out);
   __syncthreads();
   emit_exchange<R, true>(out, lds, lane, w);
   __syncthreads();
   for (uint32_t n1 = threadIdx.x; n1 < a.n1; n1 += blockDim.x) {
-    float2* dst = a.t_inv + (static_cast<size_t>(n1) * a.k0 + k0) * a.coils + chunk * 8;
+    float2* dst = a.t_inv + (static_cast<size_t>(n1) * a.k0 + k0) * a.cstride + chunk * 8;
     if (cw == 8) {
       float4* d4 = reinterpret_cast<float4*>(dst);
 #pragma unroll
@@ -195,7 +208,9 @@ __global__ __launch_bounds__(512) void inv_axis1_kernel(FftArgs a) {
 // SENSE combine over all coils, * apod * out_scale, + lambda * reg, and the
 // block's partial Re<reg, out> (CG denominator) in float64.
 template <int R>
-__global__ __launch_bounds__(512) void inv_axis0_kernel(FftArgs a) {
+__global__ __launch_bounds__(512) void inv_axis0_kernel(FftArgs args) {
+  const FftArgs a = problem_view(args, blockIdx.z);
+  const WaveTwiddles<R> twp = load_twiddles<R>(a.tw0, static_cast<int>(threadIdx.x & 63));
   extern __shared__ float2 lds[];
   __shared__ double red[512];
   constexpr int K = 64 * R;
@@ -205,11 +220,7 @@ __global__ __launch_bounds__(512) void inv_axis0_kernel(FftArgs a) {
   const uint32_t chunks = (a.coils + 7) / 8;
   for (uint32_t chunk = 0; chunk < chunks; ++chunk) {
     const uint32_t cw = min(8u, a.coils - chunk * 8);
-    for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(K) * 8; e += blockDim.x) {
-      const uint32_t k0 = e >> 3, c = e & 7;
-      lds[xslot(k0, c)] = c < cw ? a.t_inv[(static_cast<size_t>(n1) * K + k0) * a.coils + chunk * 8 + c]
-                                 : make_float2(0.f, 0.f);
-    }
+    fill_exchange<K>(a.t_inv + static_cast<size_t>(n1) * K * a.cstride + chunk * 8, a.cstride, cw, lds);
     __syncthreads();
     float2 x[R];
 #pragma unroll
@@ -217,10 +228,10 @@ __global__ __launch_bounds__(512) void inv_axis0_kernel(FftArgs a) {
     __syncthreads();
     constexpr int TPL = (R * 8 + 63) / 64;
     float2 out[TPL][8];
-    wave_fft<R, true, false, true>(x, lds + w * wave_lds_elems<R>(), a.tw0, lane, out);
+    wave_fft<R, true, false, true>(x, lds + w * wave_lds_elems<R>(), a.tw0, twp, lane, out);
     // multiply by conj(S_c[n0]) before the coil sum
     const uint32_t c = chunk * 8 + w;
-    const float2* sm = a.smaps_t + (static_cast<size_t>(n1) * a.coils + min(c, a.coils - 1)) * a.n0;
+    const float2* sm = a.smaps_t + (static_cast<size_t>(n1) * a.cstride + min(c, a.coils - 1)) * a.n0;
 #pragma unroll
     for (int s = 0; s < TPL; ++s) {
       const int task = lane + 64 * s;
@@ -297,31 +308,31 @@ static int radix_of(uint32_t k) {
     default: return fail(TSN_ERR_ARGUMENT, "unsupported FFT length"); \
   }
 
-int fft_forward(const FftArgs& a, int out_kind, hipStream_t s) {
+int fft_forward(const FftArgs& a, int out_kind, hipStream_t s, uint32_t problems) {
   const int r0 = radix_of(a.k0), r1 = radix_of(a.k1);
-  const uint32_t chunks = (a.coils + 7) / 8;
-  TSN_DISPATCH_R(r0, hipLaunchKernelGGL(fwd_axis0_kernel<R>, dim3(a.n1, chunks), dim3(512),
+  const uint32_t chunks = (a.cstride + 7) / 8;
+  TSN_DISPATCH_R(r0, hipLaunchKernelGGL(fwd_axis0_kernel<R>, dim3(a.n1, chunks, problems), dim3(512),
                                         lds_bytes_for<R>(), s, a));
   if (out_kind == 0) {
-    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 0>), dim3(a.k0, chunks), dim3(512),
+    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 0>), dim3(a.k0, chunks, problems), dim3(512),
                                           lds_bytes_for<R>(), s, a));
   } else if (out_kind == 2) {
-    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 2>), dim3(a.k0, chunks), dim3(512),
+    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 2>), dim3(a.k0, chunks, problems), dim3(512),
                                           lds_bytes_for<R>(), s, a));
   } else {
-    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 1>), dim3(a.k0, chunks), dim3(512),
+    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 1>), dim3(a.k0, chunks, problems), dim3(512),
                                           lds_bytes_for<R>(), s, a));
   }
   TSN_HIP(hipGetLastError());
   return TSN_OK;
 }
 
-int fft_inverse(const FftArgs& a, hipStream_t s) {
+int fft_inverse(const FftArgs& a, hipStream_t s, uint32_t problems) {
   const int r0 = radix_of(a.k0), r1 = radix_of(a.k1);
-  const uint32_t chunks = (a.coils + 7) / 8;
-  TSN_DISPATCH_R(r1, hipLaunchKernelGGL(inv_axis1_kernel<R>, dim3(a.k0, chunks), dim3(512),
+  const uint32_t chunks = (a.cstride + 7) / 8;
+  TSN_DISPATCH_R(r1, hipLaunchKernelGGL(inv_axis1_kernel<R>, dim3(a.k0, chunks, problems), dim3(512),
                                         lds_bytes_for<R>(), s, a));
-  TSN_DISPATCH_R(r0, hipLaunchKernelGGL(inv_axis0_kernel<R>, dim3(a.n1), dim3(512),
+  TSN_DISPATCH_R(r0, hipLaunchKernelGGL(inv_axis0_kernel<R>, dim3(a.n1, 1, problems), dim3(512),
                                         lds_bytes_for<R>(), s, a));
   TSN_HIP(hipGetLastError());
   return TSN_OK;

@@ -11,7 +11,8 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 struct FftArgs {
   uint32_t n0, n1;         // image shape
   uint32_t k0, k1;         // grid shape
-  uint32_t coils;
+  uint32_t coils;          // coils of the problem (per problem: coils_of[z] when set)
+  uint32_t cstride;        // channel stride of the [.][coil] layouts (>= every problem's coils)
   const float2* tw0;       // W_K0^k, k < K0
   const float2* tw1;       // W_K1^k
   const float2* image_t;   // [n1][n0]
@@ -39,7 +40,48 @@ struct FftArgs {
   uint32_t rr_count;
   double* rho_hist;
   uint32_t iter;
+  // Batched launches (blockIdx.z = problem z): every per-problem buffer sits
+  // z strides (elements) from the pointers above; apod_t and the twiddles are
+  // shared.  A single problem leaves them 0.
+  const uint32_t* coils_of;  // [problems] or null
+  const float2* scale_of;    // [problems] {out_scale, lambda} or null (inverse pass)
+  uint64_t image_stride;     // image_t, image_out, reg_in, cg_r, cg_p_old, cg_p_new
+  uint64_t smaps_stride;     // smaps_t
+  uint64_t t_stride;         // t_fwd, t_inv
+  uint64_t grid_stride;      // grid, panel32
+  uint64_t panel_stride;     // panel_il (uint32 words)
+  uint64_t partial_stride;   // dot_partial
+  uint64_t rr_stride;        // rr_partial
+  uint64_t hist_stride;      // rho_hist
 };
+
+// The arguments of problem z of a batched launch.
+__device__ __forceinline__ FftArgs problem_view(const FftArgs& a, uint32_t z) {
+  FftArgs v = a;
+  if (a.coils_of) v.coils = a.coils_of[z];
+  if (a.scale_of) {
+    v.out_scale = a.scale_of[z].x;
+    v.lambda = a.scale_of[z].y;
+  }
+  if (z == 0) return v;
+  const uint64_t zi = z;
+  v.image_t = a.image_t ? a.image_t + zi * a.image_stride : nullptr;
+  v.image_out = a.image_out ? a.image_out + zi * a.image_stride : nullptr;
+  v.reg_in = a.reg_in ? a.reg_in + zi * a.image_stride : nullptr;
+  v.cg_r = a.cg_r ? a.cg_r + zi * a.image_stride : nullptr;
+  v.cg_p_old = a.cg_p_old ? a.cg_p_old + zi * a.image_stride : nullptr;
+  v.cg_p_new = a.cg_p_new ? a.cg_p_new + zi * a.image_stride : nullptr;
+  v.smaps_t = a.smaps_t + zi * a.smaps_stride;
+  v.t_fwd = a.t_fwd ? a.t_fwd + zi * a.t_stride : nullptr;
+  v.t_inv = a.t_inv ? a.t_inv + zi * a.t_stride : nullptr;
+  v.grid = a.grid ? a.grid + zi * a.grid_stride : nullptr;
+  v.panel32 = a.panel32 ? a.panel32 + zi * a.grid_stride : nullptr;
+  v.panel_il = a.panel_il ? a.panel_il + zi * a.panel_stride : nullptr;
+  v.dot_partial = a.dot_partial ? a.dot_partial + zi * a.partial_stride : nullptr;
+  v.rr_partial = a.rr_partial ? a.rr_partial + zi * a.rr_stride : nullptr;
+  v.rho_hist = a.rho_hist ? a.rho_hist + zi * a.hist_stride : nullptr;
+  return v;
+}
 
 // bfloat16 hi | residual << 16 of v (both round to nearest even).
 __device__ __forceinline__ uint32_t split_bf16(float v) {
@@ -137,6 +179,29 @@ __host__ __device__ constexpr int wave_lds_elems() {
   return (R * 72 > R * 65 ? R * 72 : R * 65);
 }
 
+// Twiddles of one lane for wave_fft, loaded before the data so their latency
+// overlaps the data loads instead of stalling the passes: pass A W_K^{lane*k1}
+// (k1 < R) and pass B W_K^{R*b0*k2a} (b0 = lane & 7, k2a < 8).  R = 12 has two
+// pass-B tasks per lane and reads its twiddles in place.
+template <int R>
+struct WaveTwiddles {
+  float2 a[R];
+  float2 b[8];
+};
+
+template <int R>
+__device__ __forceinline__ WaveTwiddles<R> load_twiddles(const float2* __restrict__ tw, int lane) {
+  WaveTwiddles<R> t;
+  if constexpr (R <= 8) {
+#pragma unroll
+    for (int k1 = 1; k1 < R; ++k1) t.a[k1] = tw[lane * k1];
+    const int b0 = lane & 7;
+#pragma unroll
+    for (int k2a = 1; k2a < 8; ++k2a) t.b[k2a] = tw[R * b0 * k2a];
+  }
+  return t;
+}
+
 // Length K = 64*R DFT of one wave, R in {1, 2, 4, 8, 12}.  x[j] = element
 // lane + 64 j (elements >= K/2 are zero when HALF_IN; the caller loads zeros
 // there, and for even R the radix-R pass skips them).  Output: out[s][k2b] = X[k1 + R*(k2a + 8*k2b)]
@@ -145,7 +210,8 @@ __host__ __device__ constexpr int wave_lds_elems() {
 // waves of the block must call it together.
 template <int R, bool INV, bool HALF_IN, bool HALF_OUT>
 __device__ __forceinline__ void wave_fft(float2 (&x)[R], float2* buf, const float2* __restrict__ tw,
-                                         int lane, float2 (&out)[(R * 8 + 63) / 64][8]) {
+                                         const WaveTwiddles<R>& twp, int lane,
+                                         float2 (&out)[(R * 8 + 63) / 64][8]) {
   constexpr int K = 64 * R;
   constexpr int TPL = (R * 8 + 63) / 64;
   static_assert(R == 1 || R == 2 || R == 4 || R == 8 || R == 12, "unsupported radix");
@@ -170,7 +236,7 @@ __device__ __forceinline__ void wave_fft(float2 (&x)[R], float2* buf, const floa
     }
   }
 #pragma unroll
-  for (int k1 = 1; k1 < R; ++k1) y[k1] = twiddle<INV>(y[k1], tw[lane * k1]);
+  for (int k1 = 1; k1 < R; ++k1) y[k1] = twiddle<INV>(y[k1], R <= 8 ? twp.a[k1] : tw[lane * k1]);
 #pragma unroll
   for (int k1 = 0; k1 < R; ++k1) buf[k1 * 65 + lane] = y[k1];
   __syncthreads();
@@ -194,7 +260,7 @@ __device__ __forceinline__ void wave_fft(float2 (&x)[R], float2* buf, const floa
       float2 v[8];
       dft8<INV, 8>(u[s], v);
 #pragma unroll
-      for (int k2a = 1; k2a < 8; ++k2a) v[k2a] = twiddle<INV>(v[k2a], tw[R * b0 * k2a]);
+      for (int k2a = 1; k2a < 8; ++k2a) v[k2a] = twiddle<INV>(v[k2a], R <= 8 ? twp.b[k2a] : tw[R * b0 * k2a]);
 #pragma unroll
       for (int k2a = 0; k2a < 8; ++k2a) buf[(k1 * 8 + k2a) * 9 + b0] = v[k2a];
     }
@@ -222,6 +288,29 @@ __device__ __forceinline__ void wave_fft(float2 (&x)[R], float2* buf, const floa
 // minimum 2-way bank sharing instead of up to 16-way conflicts.
 __device__ __forceinline__ int xslot(int k, int c) { return k * 8 + (c ^ ((k ^ (k >> 3)) & 7)); }
 
+// Fill the exchange area with ROWS rows x 8 coils from global memory (row r,
+// coil c at src[r * row_stride + c]; coils >= cw read as zero).  All loads are
+// issued before the first LDS write, so a block pays one memory latency, not
+// one per loop trip.  512-thread blocks.
+template <int ROWS>
+__device__ __forceinline__ void fill_exchange(const float2* __restrict__ src, uint64_t row_stride, uint32_t cw,
+                                              float2* lds) {
+  constexpr int kElems = ROWS * 8;
+  constexpr int kPer = (kElems + 511) / 512;
+  float2 v[kPer];
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const uint32_t e = threadIdx.x + 512u * i;
+    const uint32_t r = e >> 3, c = e & 7;
+    v[i] = (e < static_cast<uint32_t>(kElems) && c < cw) ? src[r * row_stride + c] : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const uint32_t e = threadIdx.x + 512u * i;
+    if (e < static_cast<uint32_t>(kElems)) lds[xslot(static_cast<int>(e >> 3), static_cast<int>(e & 7))] = v[i];
+  }
+}
+
 // Write the wave's outputs to the block exchange area, element w of row k
 // (k < K, or k < K/2 when HALF_OUT).  Call between two __syncthreads.
 template <int R, bool HALF_OUT>
@@ -241,8 +330,9 @@ __device__ __forceinline__ void emit_exchange(const float2 (&out)[(R * 8 + 63) /
   }
 }
 
-int fft_forward(const FftArgs& a, int out_kind, hipStream_t s);
-int fft_inverse(const FftArgs& a, hipStream_t s);
+// problems > 1: batched launch over blockIdx.z (strides and coils_of in a).
+int fft_forward(const FftArgs& a, int out_kind, hipStream_t s, uint32_t problems = 1);
+int fft_inverse(const FftArgs& a, hipStream_t s, uint32_t problems = 1);
 bool fft_length_supported(uint32_t k);
 
 }  // namespace tsn

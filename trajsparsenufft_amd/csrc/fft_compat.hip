@@ -37,6 +37,7 @@ __global__ __launch_bounds__(512) void compat_fwd_rows(const float2* __restrict_
                                                        float2* __restrict__ ws) {
   extern __shared__ float2 lds[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const WaveTwiddles<kR> twp = load_twiddles<kR>(g_twiddle512, lane);
   const int c = blockIdx.y, y = blockIdx.x * 8 + w;
   float2 x[kR];
 #pragma unroll
@@ -52,7 +53,7 @@ __global__ __launch_bounds__(512) void compat_fwd_rows(const float2* __restrict_
     }
   }
   float2 out[1][8];
-  wave_fft<kR, false, true, false>(x, lds + w * wave_lds_elems<kR>(), g_twiddle512, lane, out);
+  wave_fft<kR, false, true, false>(x, lds + w * wave_lds_elems<kR>(), g_twiddle512, twp, lane, out);
   __syncthreads();
   emit_exchange<kR, false>(out, lds, lane, w);  // element row of exchange row kx
   __syncthreads();
@@ -72,13 +73,14 @@ __global__ __launch_bounds__(512) void compat_fwd_cols(const float2* __restrict_
                                                        float2* __restrict__ debug) {
   extern __shared__ float2 lds[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const WaveTwiddles<kR> twp = load_twiddles<kR>(g_twiddle512, lane);
   const int kx = blockIdx.x, c = w;
   float2 x[kR];
   const float2* col = ws + (static_cast<size_t>(c) * kG + kx) * kG;
 #pragma unroll
   for (int j = 0; j < kR; ++j) x[j] = j < kR / 2 ? col[lane + 64 * j] : make_float2(0.f, 0.f);
   float2 out[1][8];
-  wave_fft<kR, false, true, false>(x, lds + w * wave_lds_elems<kR>(), g_twiddle512, lane, out);
+  wave_fft<kR, false, true, false>(x, lds + w * wave_lds_elems<kR>(), g_twiddle512, twp, lane, out);
   __syncthreads();
   emit_exchange<kR, false>(out, lds, lane, w);  // element c of exchange row ky
   __syncthreads();
@@ -112,13 +114,14 @@ __global__ __launch_bounds__(512) void compat_inv_rows(const float2* __restrict_
                                                        float2* __restrict__ ws) {
   extern __shared__ float2 lds[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const WaveTwiddles<kR> twp = load_twiddles<kR>(g_twiddle512, lane);
   const int c = blockIdx.y, ky = blockIdx.x * 8 + w;
   const float2* row = grid + (static_cast<size_t>(c) * kG + ky) * kG;
   float2 x[kR];
 #pragma unroll
   for (int j = 0; j < kR; ++j) x[j] = row[lane + 64 * j];
   float2 out[1][8];
-  wave_fft<kR, true, false, true>(x, lds + w * wave_lds_elems<kR>(), g_twiddle512, lane, out);
+  wave_fft<kR, true, false, true>(x, lds + w * wave_lds_elems<kR>(), g_twiddle512, twp, lane, out);
   __syncthreads();
   emit_exchange<kR, true>(out, lds, lane, w);  // element row of exchange row x
   __syncthreads();
@@ -139,13 +142,14 @@ __global__ __launch_bounds__(512) void compat_inv_cols(const float2* __restrict_
                                                        float2* __restrict__ image) {
   extern __shared__ float2 lds[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const WaveTwiddles<kR> twp = load_twiddles<kR>(g_twiddle512, lane);
   const int c = blockIdx.y, x0 = blockIdx.x * 8;
   const float2* col = ws + (static_cast<size_t>(c) * kG + x0 + w) * kG;
   float2 x[kR];
 #pragma unroll
   for (int j = 0; j < kR; ++j) x[j] = col[lane + 64 * j];
   float2 out[1][8];
-  wave_fft<kR, true, false, true>(x, lds + w * wave_lds_elems<kR>(), g_twiddle512, lane, out);
+  wave_fft<kR, true, false, true>(x, lds + w * wave_lds_elems<kR>(), g_twiddle512, twp, lane, out);
   __syncthreads();
   emit_exchange<kR, true>(out, lds, lane, w);  // element col of exchange row y
   __syncthreads();

@@ -64,7 +64,92 @@ struct InterpArgs {
   _Float16* out_hi;       // split output panel blocks
   _Float16* out_res;
   uint32_t* out_il;       // interleaved split output panel blocks
+  uint32_t coil_base;     // first coil of block 0 in the output (batched launches)
+  // Batched launches (one coil block of one problem per entry; NB = 1,
+  // interleaved panels): items[i] belongs to problems[item_problem[i]].
+  const struct InterpProblem* problems;
+  const uint32_t* item_problem;
+  const uint32_t* long_problem;
+  // Data-consistency epilogue of split outputs: each value v becomes
+  // (v - dc_scale * y) * w before the split.  y is read at
+  // row * dc_row_stride + coil * dc_coil_stride + part (floats); null = off.
+  const float* dc_meas;
+  const float* dc_density;  // w per output row, or null (all ones)
+  float dc_scale;
+  uint32_t dc_row_stride, dc_coil_stride;
 };
+
+// One coil block of one problem of a batched interpolation launch.
+struct InterpProblem {
+  const int32_t* group_rows;
+  const uint32_t* tile_cols;
+  const uint4* tile_a;
+  const uint16_t* tile_idx;
+  const uint4* tile_dense;
+  uint32_t n_tiles, n_groups, n_out, n_cols;
+  uint32_t coils;       // coils of the problem (output row layouts)
+  uint32_t coil_base;   // first coil of this block
+  const uint32_t* panel_il;  // this block's input panel [n_cols][16]
+  float* out_f32;       // the problem's float2 output
+  uint32_t* out_il;     // this block's split output panel [n_out][16]
+};
+
+__device__ __forceinline__ InterpArgs problem_args(const InterpArgs& a, uint32_t vp) {
+  const InterpProblem& q = a.problems[vp];
+  InterpArgs v = a;
+  v.group_rows = q.group_rows;
+  v.tile_cols = q.tile_cols;
+  v.tile_a = q.tile_a;
+  v.tile_idx = q.tile_idx;
+  v.tile_dense = q.tile_dense;
+  v.n_tiles = q.n_tiles;
+  v.n_groups = q.n_groups;
+  v.n_out = q.n_out;
+  v.coils = q.coils;
+  v.coil_base = q.coil_base;
+  v.panel_il = q.panel_il;
+  v.panel_stride = static_cast<uint64_t>(q.n_cols) * kN;  // one block: its rows bound the buffer range
+  v.out_f32 = q.out_f32;
+  v.out_il = q.out_il;
+  return v;
+}
+
+// hi | res << 16 of one value: FP16 (RN) or BF16 (RNE) halves, res = v - hi.
+// v is pinned as an FP32 value first: otherwise the compiler may fold the
+// multiply that produced it into the FP16 conversion (v_fma_mix, one rounding
+// instead of two), and identical inputs would split differently in different
+// kernels.
+template <bool BF16>
+__device__ __forceinline__ uint32_t split_word(float v) {
+  asm volatile("" : "+v"(v));
+  if constexpr (BF16) {
+    const uint16_t hb = bf16_rne_bits(v);
+    return static_cast<uint32_t>(hb) | (static_cast<uint32_t>(bf16_rne_bits(v - bf16_to_float(hb))) << 16);
+  } else {
+    const _Float16 h = static_cast<_Float16>(v);
+    const _Float16 r = static_cast<_Float16>(v - static_cast<float>(h));
+    return static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
+           (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r)) << 16);
+  }
+}
+
+// Data-consistency residual (prediction - scale * y) * w, every step rounded
+// on its own (no contraction into fma), so that the fused epilogue and the
+// materialized pass over stored predictions produce the same bits.
+__device__ __forceinline__ float dc_residual(float pred, float y, float scale, float w) {
+#pragma clang fp contract(off)
+  const float m = y * scale;
+  return (pred - m) * w;
+}
+
+__device__ __forceinline__ float dc_apply(const InterpArgs& a, float v, uint32_t coil, int row, int part) {
+  asm volatile("" : "+v"(v));  // v is final here, as if loaded from memory
+  const float y = coil < a.coils ? a.dc_meas[static_cast<size_t>(row) * a.dc_row_stride +
+                                             static_cast<size_t>(coil) * a.dc_coil_stride + part]
+                                 : 0.0f;
+  const float w = a.dc_density ? a.dc_density[row] : 1.0f;
+  return dc_residual(v, y, a.dc_scale, w);
+}
 
 template <int OM, bool CONJ, bool ACC>
 __device__ __forceinline__ void interp_epilogue(const InterpArgs& a, const int4& rows, int b,
@@ -72,7 +157,7 @@ __device__ __forceinline__ void interp_epilogue(const InterpArgs& a, const int4&
                                                 int lane) {
   const int n = lane & 15;
   const int rr[4] = {rows.x, rows.y, rows.z, rows.w};
-  const uint32_t coil = b * kCoilsPerBlock + (n & 7);
+  const uint32_t coil = a.coil_base + b * kCoilsPerBlock + (n & 7);
   const int part = n >> 3;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -83,22 +168,16 @@ __device__ __forceinline__ void interp_epilogue(const InterpArgs& a, const int4&
     v *= a.scale;
     const int row = rr[i];
     if (row < 0) continue;
-    if constexpr (OM == kOutSplitBF16IL) {
-      const uint16_t hb = bf16_rne_bits(v);
-      const uint16_t rb = bf16_rne_bits(v - bf16_to_float(hb));
-      a.out_il[(static_cast<size_t>(b) * a.n_out + row) * kN + n] =
-          static_cast<uint32_t>(hb) | (static_cast<uint32_t>(rb) << 16);
-    } else if constexpr (OM == kOutSplit16 || OM == kOutSplit16IL) {
+    if constexpr (OM == kOutSplitBF16IL || OM == kOutSplit16IL) {
+      if (a.dc_meas) v = dc_apply(a, v, coil, row, part);
+      a.out_il[(static_cast<size_t>(b) * a.n_out + row) * kN + n] = split_word<OM == kOutSplitBF16IL>(v);
+    } else if constexpr (OM == kOutSplit16) {
+      asm volatile("" : "+v"(v));  // split the FP32 value (see split_word)
       const _Float16 hi = static_cast<_Float16>(v);
       const _Float16 res = static_cast<_Float16>(v - static_cast<float>(hi));
       const size_t o = (static_cast<size_t>(b) * a.n_out + row) * kN + n;
-      if constexpr (OM == kOutSplit16IL) {
-        a.out_il[o] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, hi)) |
-                      (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, res)) << 16);
-      } else {
-        a.out_hi[o] = hi;
-        a.out_res[o] = res;
-      }
+      a.out_hi[o] = hi;
+      a.out_res[o] = res;
     } else {
       if (coil >= a.coils) continue;
       size_t o;
@@ -353,24 +432,13 @@ __device__ __forceinline__ void stream_tiles(const InterpArgs& a, const TileRsrc
   }
 }
 
-// Blocks of kTileWaves waves.  The first n_long blocks each own one long group
-// (waves stride over its tiles and are summed through LDS in wave order); the
-// other blocks hold one work unit (a run of whole groups) per wave, the units
-// of one XCD contiguous in pack order.
+// A long group: the block's waves stride over its tiles; partial sums are
+// added through LDS in wave order (deterministic), wave 0 runs the epilogue.
 template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC>
-__global__ __launch_bounds__(kTileWaves * 64) void interp_tiles_kernel(InterpArgs a) {
-  __shared__ float4_t red[kTileWaves][NB][2][64];
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-  float4_t acc_re[NB], acc_im[NB];
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    acc_re[b] = float4_t{0.f, 0.f, 0.f, 0.f};
-    acc_im[b] = float4_t{0.f, 0.f, 0.f, 0.f};
-  }
-  if (blockIdx.x < a.n_long) {
-    const WorkItem it = a.long_items[blockIdx.x];
-    if (it.group < 0) return;  // padding block: uniform exit before the barrier
+__device__ __forceinline__ void long_group(const InterpArgs& a, const WorkItem& it,
+                                           float4_t (&red)[kTileWaves][NB][2][64], float4_t (&acc_re)[NB],
+                                           float4_t (&acc_im)[NB], int lane, int wave) {
+  {
     const TileRsrc<NB> r = make_tile_rsrc<NB, IL>(a);
     stream_tiles<KIND, NB, IL, false, OM, CONJ, ACC>(a, r, it.tile_begin + wave, it.tile_end, kTileWaves,
                                                      it.group, it.group_end, acc_re, acc_im, lane);
@@ -391,12 +459,13 @@ __global__ __launch_bounds__(kTileWaves * 64) void interp_tiles_kernel(InterpArg
       }
       interp_epilogue<OM, CONJ, ACC>(a, rows, b, re, im, lane);
     }
-    return;
   }
-  const uint32_t rb = blockIdx.x - a.n_long;
-  const uint32_t item_id = xcd_remap(rb, gridDim.x - a.n_long) * kTileWaves + wave;
-  if (item_id >= a.n_items) return;
-  const WorkItem u = a.items[item_id];
+}
+
+// A regular unit: one wave streams its whole groups.
+template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC>
+__device__ __forceinline__ void regular_unit(const InterpArgs& a, const WorkItem& u, float4_t (&acc_re)[NB],
+                                             float4_t (&acc_im)[NB], int lane) {
   if (u.tile_begin == u.tile_end) {  // empty group: zero rows
     const int4 rows = group_rows_of(a, static_cast<uint32_t>(u.group), lane);
 #pragma unroll
@@ -407,6 +476,49 @@ __global__ __launch_bounds__(kTileWaves * 64) void interp_tiles_kernel(InterpArg
   stream_tiles<KIND, NB, IL, true, OM, CONJ, ACC>(a, r, u.tile_begin, u.tile_end, 1, u.group, u.group_end,
                                                   acc_re, acc_im, lane);
 }
+
+// Blocks of kTileWaves waves.  The first n_long blocks each own one long group
+// (waves stride over its tiles and are summed through LDS in wave order); the
+// other blocks hold one work unit (a run of whole groups) per wave, the units
+// of one XCD contiguous in pack order.
+template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC>
+__global__ __launch_bounds__(kTileWaves * 64) void interp_tiles_kernel(InterpArgs a) {
+  __shared__ float4_t red[kTileWaves][NB][2][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  float4_t acc_re[NB], acc_im[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    acc_re[b] = float4_t{0.f, 0.f, 0.f, 0.f};
+    acc_im[b] = float4_t{0.f, 0.f, 0.f, 0.f};
+  }
+  if (blockIdx.x < a.n_long) {
+    const WorkItem it = a.long_items[blockIdx.x];
+    if (it.group < 0) return;  // padding block: uniform exit before the barrier
+    if constexpr (NB == 1 && IL) {
+      if (a.problems) {
+        const InterpArgs v = problem_args(a, a.long_problem[blockIdx.x]);
+        long_group<KIND, NB, IL, OM, CONJ, ACC>(v, it, red, acc_re, acc_im, lane, wave);
+        return;
+      }
+    }
+    long_group<KIND, NB, IL, OM, CONJ, ACC>(a, it, red, acc_re, acc_im, lane, wave);
+    return;
+  }
+  const uint32_t rb = blockIdx.x - a.n_long;
+  const uint32_t item_id = xcd_remap(rb, gridDim.x - a.n_long) * kTileWaves + wave;
+  if (item_id >= a.n_items) return;
+  const WorkItem u = a.items[item_id];
+  if constexpr (NB == 1 && IL) {
+    if (a.problems) {
+      regular_unit<KIND, NB, IL, OM, CONJ, ACC>(problem_args(a, a.item_problem[item_id]), u, acc_re, acc_im,
+                                                lane);
+      return;
+    }
+  }
+  regular_unit<KIND, NB, IL, OM, CONJ, ACC>(a, u, acc_re, acc_im, lane);
+}
+
 
 struct CsrArgs {
   const uint32_t* row_ptr;
@@ -534,6 +646,7 @@ __global__ __launch_bounds__(256) void split_panel_kernel(const float2* __restri
     const float2 x = in[static_cast<size_t>(coil) * rows + row];
     v = (n < 8 ? x.x : x.y) * scale;
   }
+  asm volatile("" : "+v"(v));  // split the FP32 value (see split_word)
   const _Float16 h = static_cast<_Float16>(v);
   hi[gtid] = h;
   res[gtid] = static_cast<_Float16>(v - static_cast<float>(h));
@@ -558,15 +671,34 @@ __global__ __launch_bounds__(256) void split_panel_il_kernel(const float2* __res
     const float2 x = in[static_cast<size_t>(coil) * rows + row];
     v = (n < 8 ? x.x : x.y) * scale;
   }
-  if constexpr (BF16) {
-    const uint16_t hb = bf16_rne_bits(v);
-    out[gtid] = static_cast<uint32_t>(hb) | (static_cast<uint32_t>(bf16_rne_bits(v - bf16_to_float(hb))) << 16);
-  } else {
-    const _Float16 h = static_cast<_Float16>(v);
-    const _Float16 r = static_cast<_Float16>(v - static_cast<float>(h));
-    out[gtid] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
-                (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r)) << 16);
+  out[gtid] = split_word<BF16>(v);
+}
+
+// Materialized data consistency: out = split((pred - scale * y) * w) with
+// pred float2 [rows][coils] (a row-major interpolation output) and y read
+// through the same strides as the fused epilogue.  One thread per panel word.
+template <bool BF16>
+__global__ __launch_bounds__(256) void dc_residual_kernel(const float* __restrict__ pred, const float* __restrict__ y,
+                                                          const float* __restrict__ density, uint32_t rows,
+                                                          uint32_t coils, uint32_t blocks, float scale,
+                                                          uint32_t row_stride, uint32_t coil_stride,
+                                                          uint32_t* __restrict__ out) {
+  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t total = static_cast<uint64_t>(blocks) * rows * kN;
+  if (gtid >= total) return;
+  const int n = static_cast<int>(gtid & 15);
+  const uint64_t br = gtid >> 4;
+  const uint32_t row = static_cast<uint32_t>(br % rows);
+  const uint32_t b = static_cast<uint32_t>(br / rows);
+  const uint32_t coil = b * kCoilsPerBlock + (n & 7);
+  const int part = n >> 3;
+  float p = 0.0f, m = 0.0f;
+  if (coil < coils) {
+    p = pred[(static_cast<size_t>(row) * coils + coil) * 2 + part];
+    m = y[static_cast<size_t>(row) * row_stride + static_cast<size_t>(coil) * coil_stride + part];
   }
+  const float w = density ? density[row] : 1.0f;
+  out[gtid] = split_word<BF16>(dc_residual(p, m, scale, w));
 }
 
 // ------------------------------------------------------------- dispatch
@@ -620,12 +752,10 @@ static void launch_tiles(const InterpArgs& a, bool il, int om, bool conj, bool a
   }
 }
 
-// Internal entry used by the plan and the C ABI.  All pointers are device
-// pointers sized by the caller; panels hold ceil(coils/8) blocks.
-int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* res, uint32_t coils,
-                  int out_mode, bool conj, bool accumulate, float scale, float* out_f32,
-                  _Float16* out_hi, _Float16* out_res, hipStream_t stream, bool interleaved,
-                  uint32_t* out_il) {
+static int launch_interp_impl(const tsn_direction* d, int kind, const void* hi, const void* res, uint32_t coils,
+                              int out_mode, bool conj, bool accumulate, float scale, float* out_f32,
+                              _Float16* out_hi, _Float16* out_res, hipStream_t stream, bool interleaved,
+                              uint32_t* out_il, const DcTerms* dc) {
   if (!d || d->csr) return fail(TSN_ERR_ARGUMENT, "interpolation needs a packed (non-CSR) direction");
   if (coils == 0 || coils > static_cast<uint32_t>(kMaxCoils))
     return fail(TSN_ERR_ARGUMENT, "coils must be in 1..16 (got %u)", coils);
@@ -666,11 +796,81 @@ int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* 
   a.out_hi = out_hi;
   a.out_res = out_res;
   a.out_il = out_il;
+  if (dc) {
+    a.dc_meas = dc->y;
+    a.dc_density = dc->density;
+    a.dc_scale = dc->scale;
+    a.dc_row_stride = dc->row_stride;
+    a.dc_coil_stride = dc->coil_stride;
+  }
   if (kind == kSparseSplit) launch_tiles<kSparseSplit>(a, interleaved, out_mode, conj, accumulate, stream);
   else if (kind == kDenseSplit) launch_tiles<kDenseSplit>(a, interleaved, out_mode, conj, accumulate, stream);
   else if (kind == kSparseSplitBF16)
     launch_tiles<kSparseSplitBF16>(a, interleaved, out_mode, conj, accumulate, stream);
   else launch_tiles<kSparseHi>(a, interleaved, out_mode, conj, accumulate, stream);
+  TSN_HIP(hipGetLastError());
+  return TSN_OK;
+}
+
+// Internal entry used by the plan and the C ABI.  All pointers are device
+// pointers sized by the caller; panels hold ceil(coils/8) blocks.
+int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* res, uint32_t coils,
+                  int out_mode, bool conj, bool accumulate, float scale, float* out_f32,
+                  _Float16* out_hi, _Float16* out_res, hipStream_t stream, bool interleaved,
+                  uint32_t* out_il) {
+  return launch_interp_impl(d, kind, hi, res, coils, out_mode, conj, accumulate, scale, out_f32, out_hi, out_res,
+                            stream, interleaved, out_il, nullptr);
+}
+
+int launch_interp_dc(const tsn_direction* d, int kind, const uint32_t* panel, uint32_t coils, float scale,
+                     const DcTerms& dc, uint32_t* out_il, hipStream_t s) {
+  if (!dc.y) return fail(TSN_ERR_ARGUMENT, "missing measurements");
+  return launch_interp_impl(d, kind, panel, nullptr, coils, kOutSplit16IL, false, false, scale, nullptr, nullptr,
+                            nullptr, s, true, out_il, &dc);
+}
+
+int launch_dc_residual(const float2* pred, uint32_t rows, uint32_t coils, const DcTerms& dc, bool bf16,
+                       uint32_t* out_il, hipStream_t s) {
+  if (!pred || !dc.y || !out_il) return fail(TSN_ERR_ARGUMENT, "null argument");
+  if (coils == 0 || coils > static_cast<uint32_t>(kMaxCoils)) return fail(TSN_ERR_ARGUMENT, "coils must be in 1..16");
+  const uint32_t blocks = div_up(coils, kCoilsPerBlock);
+  const uint64_t total = static_cast<uint64_t>(blocks) * rows * kN;
+  if (total == 0) return TSN_OK;
+  const dim3 grid(static_cast<uint32_t>(div_up(total, 256)));
+  const float* p = reinterpret_cast<const float*>(pred);
+  if (bf16)
+    hipLaunchKernelGGL(dc_residual_kernel<true>, grid, dim3(256), 0, s, p, dc.y, dc.density, rows, coils, blocks,
+                       dc.scale, dc.row_stride, dc.coil_stride, out_il);
+  else
+    hipLaunchKernelGGL(dc_residual_kernel<false>, grid, dim3(256), 0, s, p, dc.y, dc.density, rows, coils, blocks,
+                       dc.scale, dc.row_stride, dc.coil_stride, out_il);
+  TSN_HIP(hipGetLastError());
+  return TSN_OK;
+}
+
+// FP32 rows float2 [rows][coils] in place: v <- (v - scale * y) * w.
+__global__ __launch_bounds__(256) void dc_residual_rows_kernel(float* __restrict__ io, const float* __restrict__ y,
+                                                               const float* __restrict__ density, uint32_t rows,
+                                                               uint32_t coils, float scale, uint32_t row_stride,
+                                                               uint32_t coil_stride) {
+  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t per_row = 2ull * coils;
+  if (gtid >= per_row * rows) return;
+  const uint32_t row = static_cast<uint32_t>(gtid / per_row);
+  const uint32_t rem = static_cast<uint32_t>(gtid % per_row);
+  const uint32_t coil = rem >> 1, part = rem & 1u;
+  const float w = density ? density[row] : 1.0f;
+  io[gtid] = dc_residual(io[gtid], y[static_cast<size_t>(row) * row_stride + static_cast<size_t>(coil) * coil_stride + part],
+                         scale, w);
+}
+
+int launch_dc_residual_rows(float2* rows_io, uint32_t rows, uint32_t coils, const DcTerms& dc, hipStream_t s) {
+  if (!rows_io || !dc.y) return fail(TSN_ERR_ARGUMENT, "null argument");
+  const uint64_t total = 2ull * coils * rows;
+  if (total == 0) return TSN_OK;
+  hipLaunchKernelGGL(dc_residual_rows_kernel, dim3(static_cast<uint32_t>(div_up(total, 256))), dim3(256), 0, s,
+                     reinterpret_cast<float*>(rows_io), dc.y, dc.density, rows, coils, dc.scale, dc.row_stride,
+                     dc.coil_stride);
   TSN_HIP(hipGetLastError());
   return TSN_OK;
 }
@@ -841,6 +1041,8 @@ static int schedule_units(tsn_direction* d, uint32_t unit_tiles) {
   d->n_items = static_cast<uint32_t>(items.size());
   d->n_long = static_cast<uint32_t>(long_items.size());
   d->unit_tiles = static_cast<uint32_t>(target);
+  d->host_items = items;
+  d->host_long = long_items;
   d->bytes += bytes;
   return TSN_OK;
 }
@@ -923,6 +1125,124 @@ static int upload_pack(const HostPack& p, tsn_direction** out) {
   return TSN_OK;
 }
 
+int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* out) {
+  *out = InterpBatch{};
+  if (parts.empty()) return fail(TSN_ERR_ARGUMENT, "empty interpolation batch");
+  const int values = parts[0].dir->values;
+  std::vector<InterpProblem> probs;
+  for (const InterpBatchPart& p : parts) {
+    const tsn_direction* d = p.dir;
+    if (!d || d->csr) return fail(TSN_ERR_ARGUMENT, "batched interpolation needs packed directions");
+    if (d->values != values) return fail(TSN_ERR_ARGUMENT, "a batch mixes tile value formats");
+    InterpProblem q{};
+    q.group_rows = d->group_rows;
+    q.tile_cols = d->tile_cols;
+    q.tile_a = d->tile_a;
+    q.tile_idx = d->tile_idx;
+    q.tile_dense = d->tile_dense;
+    q.n_tiles = d->h.tiles;
+    q.n_groups = d->h.groups;
+    q.n_out = d->h.output_size;
+    q.n_cols = d->h.n_cols;
+    q.coils = p.coils;
+    q.coil_base = p.coil_base;
+    q.panel_il = p.panel_il;
+    q.out_f32 = p.out_f32;
+    q.out_il = p.out_il;
+    probs.push_back(q);
+  }
+  // parts grouped by direction, in order of first appearance; unit-major within a group
+  std::vector<const tsn_direction*> dirs;
+  for (const InterpBatchPart& p : parts)
+    if (std::find(dirs.begin(), dirs.end(), p.dir) == dirs.end()) dirs.push_back(p.dir);
+  std::vector<WorkItem> items, long_items;
+  std::vector<uint32_t> item_problem, long_problem;
+  for (const tsn_direction* d : dirs) {
+    std::vector<uint32_t> members;
+    for (uint32_t i = 0; i < parts.size(); ++i)
+      if (parts[i].dir == d) members.push_back(i);
+    for (const WorkItem& u : d->host_items)
+      for (uint32_t m : members) {
+        items.push_back(u);
+        item_problem.push_back(m);
+      }
+    for (const WorkItem& l : d->host_long) {
+      if (l.group < 0) continue;  // the direction's own padding
+      for (uint32_t m : members) {
+        long_items.push_back(l);
+        long_problem.push_back(m);
+      }
+    }
+  }
+  while (long_items.size() % 8) {
+    long_items.push_back({-1, -1, 0, 0});
+    long_problem.push_back(0);
+  }
+  InterpBatch b;
+  b.values = values;
+  b.dense = std::all_of(parts.begin(), parts.end(), [](const InterpBatchPart& p) { return p.dir->tile_dense; });
+  b.n_problems = static_cast<uint32_t>(probs.size());
+  b.n_items = static_cast<uint32_t>(items.size());
+  b.n_long = static_cast<uint32_t>(long_items.size());
+  size_t bytes = 0;
+  InterpProblem* dprobs = nullptr;
+  int rc = device_copy(probs.data(), probs.size(), &dprobs, &bytes);
+  b.problems = dprobs;
+  if (rc == TSN_OK) rc = device_copy(items.data(), items.size(), &b.items, &bytes);
+  if (rc == TSN_OK) rc = device_copy(item_problem.data(), item_problem.size(), &b.item_problem, &bytes);
+  if (rc == TSN_OK) rc = device_copy(long_items.data(), long_items.size(), &b.long_items, &bytes);
+  if (rc == TSN_OK) rc = device_copy(long_problem.data(), long_problem.size(), &b.long_problem, &bytes);
+  if (rc != TSN_OK) {
+    interp_batch_free(&b);
+    return rc;
+  }
+  *out = b;
+  return TSN_OK;
+}
+
+void interp_batch_free(InterpBatch* b) {
+  if (!b) return;
+  void* ptrs[] = {b->problems, b->items, b->item_problem, b->long_items, b->long_problem};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  *b = InterpBatch{};
+}
+
+template <int KIND>
+static void launch_batch_kind(const InterpArgs& a, int out_mode, bool conj, hipStream_t s) {
+  if (out_mode == kOutRowMajor) launch_tiles_om<KIND, 1, true, kOutRowMajor>(a, conj, false, s);
+  else if (KIND == kSparseSplitBF16) launch_tiles_om<KIND, 1, true, kOutSplitBF16IL>(a, conj, false, s);
+  else launch_tiles_om<KIND, 1, true, kOutSplit16IL>(a, conj, false, s);
+}
+
+int launch_interp_batch(const InterpBatch& b, int kind, int out_mode, bool conj, float scale, hipStream_t s) {
+  if (!b.problems) return fail(TSN_ERR_ARGUMENT, "empty interpolation batch");
+  if (out_mode != kOutRowMajor && out_mode != kOutSplit16IL)
+    return fail(TSN_ERR_ARGUMENT, "batched interpolation writes row-major float2 or split panels");
+  if ((kind == kSparseSplitBF16) != (b.values == TSN_VALUES_BF16))
+    return fail(TSN_ERR_ARGUMENT, "tile value format of the batch does not match the interpolation variant");
+  if (kind == kDenseSplit && !b.dense) return fail(TSN_ERR_ARGUMENT, "batch has no dense-control tiles");
+  InterpArgs a{};
+  a.items = b.items;
+  a.n_items = b.n_items;
+  a.long_items = b.long_items;
+  a.n_long = b.n_long;
+  a.scale = scale;
+  a.problems = static_cast<const InterpProblem*>(b.problems);
+  a.item_problem = b.item_problem;
+  a.long_problem = b.long_problem;
+  a.coils = kCoilsPerBlock;  // replaced per problem
+  switch (kind) {
+    case kSparseSplit: launch_batch_kind<kSparseSplit>(a, out_mode, conj, s); break;
+    case kDenseSplit: launch_batch_kind<kDenseSplit>(a, out_mode, conj, s); break;
+    case kSparseHi: launch_batch_kind<kSparseHi>(a, out_mode, conj, s); break;
+    case kSparseSplitBF16: launch_batch_kind<kSparseSplitBF16>(a, out_mode, conj, s); break;
+    default: return fail(TSN_ERR_ARGUMENT, "unknown interpolation kind %d", kind);
+  }
+  TSN_HIP(hipGetLastError());
+  return TSN_OK;
+}
+
 }  // namespace tsn
 
 extern "C" {
@@ -943,6 +1263,66 @@ int tsn_direction_schedule(tsn_direction* d, uint32_t unit_tiles) {
 }
 
 uint32_t tsn_direction_unit_tiles(const tsn_direction* d) { return d ? d->unit_tiles : 0; }
+
+int tsn_direction_autotune(tsn_direction* d, uint32_t coils, void* stream, uint32_t* best_unit_tiles,
+                           float* best_us) {
+  if (!d) return fail(TSN_ERR_ARGUMENT, "null direction");
+  if (d->csr) return fail(TSN_ERR_ARGUMENT, "a CSR direction has no tile schedule");
+  if (coils == 0 || coils > static_cast<uint32_t>(kMaxCoils)) return fail(TSN_ERR_ARGUMENT, "coils must be in 1..16");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  TSN_HIP(hipSetDevice(d->device));
+  const uint32_t blocks = div_up(coils, kCoilsPerBlock);
+  const size_t panel_words = static_cast<size_t>(blocks) * d->h.n_cols * kN;
+  const size_t out_floats = static_cast<size_t>(coils) * d->h.output_size * 2;
+  uint32_t* panel = nullptr;
+  float* out = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = TSN_OK;
+  auto hip = [&](hipError_t e, const char* what) {
+    if (rc == TSN_OK && e != hipSuccess) rc = fail(TSN_ERR_RUNTIME, "autotune %s: %s", what, hipGetErrorString(e));
+  };
+  hip(hipMalloc(reinterpret_cast<void**>(&panel), panel_words * 4), "alloc");
+  hip(hipMalloc(reinterpret_cast<void**>(&out), out_floats * 4), "alloc");
+  if (rc == TSN_OK) hip(hipMemsetAsync(panel, 0, panel_words * 4, s), "clear");
+  hip(hipEventCreate(&e0), "event");
+  hip(hipEventCreate(&e1), "event");
+  const int kind = d->values == TSN_VALUES_BF16 ? kSparseSplitBF16 : kSparseSplit;
+  const uint32_t candidates[] = {2, 3, 4, 5, 6, 8, 10, 12, 16};
+  uint32_t best = 0;
+  float best_ms = 0.0f;
+  for (uint32_t u : candidates) {
+    if (rc != TSN_OK) break;
+    rc = schedule_units(d, u);
+    auto launch = [&]() {
+      return launch_interp(d, kind, panel, nullptr, coils, kOutCoilMajor, false, false, 1.0f, out, nullptr,
+                           nullptr, s, true, nullptr);
+    };
+    for (int i = 0; rc == TSN_OK && i < 3; ++i) rc = launch();
+    if (rc != TSN_OK) break;
+    hip(hipEventRecord(e0, s), "event");
+    constexpr int kReps = 10;
+    for (int i = 0; rc == TSN_OK && i < kReps; ++i) rc = launch();
+    hip(hipEventRecord(e1, s), "event");
+    hip(hipEventSynchronize(e1), "event");
+    float ms = 0.0f;
+    hip(hipEventElapsedTime(&ms, e0, e1), "event");
+    ms /= kReps;
+    if (rc == TSN_OK && (best == 0 || ms < best_ms)) {
+      best = u;
+      best_ms = ms;
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (panel) (void)hipFree(panel);
+  if (out) (void)hipFree(out);
+  if (rc == TSN_OK) rc = schedule_units(d, best);
+  if (rc != TSN_OK) return rc;
+  if (best_unit_tiles) *best_unit_tiles = best;
+  if (best_us) *best_us = best_ms * 1000.0f;
+  return TSN_OK;
+}
 
 int tsn_direction_load(const char* real_dir, const char* imag_dir, tsn_direction** out) {
   if (!out) return fail(TSN_ERR_ARGUMENT, "null argument");
@@ -1081,6 +1461,34 @@ int tsn_interp_il(const tsn_direction* d, const void* panel, void* out, uint32_t
   return launch_interp(d, d->values == TSN_VALUES_BF16 ? kSparseSplitBF16 : kSparseSplit, panel, nullptr, coils,
                        out_layout, conjugate != 0, accumulate != 0, scale, static_cast<float*>(out), nullptr,
                        nullptr, static_cast<hipStream_t>(stream), true, nullptr);
+}
+
+int tsn_interp_dc(const tsn_direction* d, const void* panel, const void* measurements, int measurement_layout,
+                  const float* density, uint32_t coils, float scale, float measurement_scale, int mode,
+                  void* workspace, void* out_panel, void* stream) {
+  if (!d || !panel || !measurements || !out_panel) return fail(TSN_ERR_ARGUMENT, "null argument");
+  if (d->csr) return fail(TSN_ERR_ARGUMENT, "data consistency needs a packed (non-CSR) direction");
+  if (coils == 0 || coils > static_cast<uint32_t>(kMaxCoils)) return fail(TSN_ERR_ARGUMENT, "coils must be in 1..16");
+  if (measurement_layout != TSN_OUT_COIL_MAJOR && measurement_layout != TSN_OUT_ROW_MAJOR)
+    return fail(TSN_ERR_ARGUMENT, "unknown measurement layout %d", measurement_layout);
+  if (mode != TSN_DC_MATERIALIZED && mode != TSN_DC_FUSED)
+    return fail(TSN_ERR_ARGUMENT, "data-consistency mode must be 1 (materialized) or 2 (fused), got %d", mode);
+  if (mode == TSN_DC_MATERIALIZED && !workspace) return fail(TSN_ERR_ARGUMENT, "materialized mode needs a workspace");
+  const uint32_t rows = d->h.output_size;
+  DcTerms dc;
+  dc.y = static_cast<const float*>(measurements);
+  dc.density = density;
+  dc.scale = measurement_scale;
+  dc.row_stride = measurement_layout == TSN_OUT_ROW_MAJOR ? 2 * coils : 2;
+  dc.coil_stride = measurement_layout == TSN_OUT_ROW_MAJOR ? 2 : 2 * rows;
+  const bool bf16 = d->values == TSN_VALUES_BF16;
+  const int kind = bf16 ? kSparseSplitBF16 : kSparseSplit;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto* out = static_cast<uint32_t*>(out_panel);
+  if (mode == TSN_DC_FUSED) return launch_interp_dc(d, kind, static_cast<const uint32_t*>(panel), coils, scale, dc, out, s);
+  TSN_CHECK(launch_interp(d, kind, panel, nullptr, coils, kOutRowMajor, false, false, scale,
+                          static_cast<float*>(workspace), nullptr, nullptr, s, true, nullptr));
+  return launch_dc_residual(static_cast<const float2*>(workspace), rows, coils, dc, bf16, out, s);
 }
 
 int tsn_split_panel_il(const void* in, uint32_t rows, uint32_t coils, float scale, int values, void* panel,

@@ -36,156 +36,7 @@ int launch_split_panel(const float2* in, uint32_t rows, uint32_t coils, float sc
 
 using namespace tsn;
 
-namespace {
-
-constexpr int kVecThreads = 256;
-
-__device__ __forceinline__ double block_sum(double v, double* red) {
-  red[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-    if (static_cast<int>(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
-    __syncthreads();
-  }
-  return red[0];
-}
-
-// Fixed-order sum of partials: every block that calls it gets the same value.
-__device__ __forceinline__ double sum_partials(const double* __restrict__ partial, uint32_t count, double* red) {
-  double s = 0.0;
-  for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) s += partial[i];
-  return block_sum(s, red);
-}
-
-// x = 0, r = p = b, partial <r, r>
-__global__ __launch_bounds__(kVecThreads) void cg_init_kernel(const float2* __restrict__ b,
-                                                              float2* __restrict__ x, float2* __restrict__ r,
-                                                              float2* __restrict__ p, uint32_t n,
-                                                              double* __restrict__ partial) {
-  __shared__ double red[kVecThreads];
-  double s = 0.0;
-  const uint32_t e = blockIdx.x * kVecThreads + threadIdx.x;
-  if (e < n) {
-    const float2 v = b[e];
-    x[e] = make_float2(0.f, 0.f);
-    r[e] = v;
-    p[e] = v;
-    s = static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y;
-  }
-  s = block_sum(s, red);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
-
-// alpha = rho_k / Re<p, Ap> (0 and status 0 on non-positive or non-finite
-// curvature), x += alpha p, r -= alpha Ap, partial <r, r>.  rho_k is
-// rho_hist[iter]; every block reduces the same <p, Ap> partials itself.
-__global__ __launch_bounds__(kVecThreads) void cg_update_xr_kernel(
-    float2* __restrict__ x, float2* __restrict__ r, const float2* __restrict__ p,
-    const float2* __restrict__ ap, const double* __restrict__ dot_partial, uint32_t dot_count,
-    const double* __restrict__ rho_hist, uint32_t iter, int* __restrict__ status, uint32_t n,
-    double* __restrict__ rr_partial) {
-  __shared__ double red[kVecThreads];
-  const double den = sum_partials(dot_partial, dot_count, red);
-  __syncthreads();
-  double alpha = 0.0;
-  if (isfinite(den) && den > 0.0) alpha = rho_hist[iter] / den;
-  else if (blockIdx.x == 0 && threadIdx.x == 0) *status = 0;
-  double s = 0.0;
-  const uint32_t e = blockIdx.x * kVecThreads + threadIdx.x;
-  if (e < n) {
-    const float2 pv = p[e], av = ap[e];
-    float2 xv = x[e], rv = r[e];
-    xv.x = static_cast<float>(xv.x + alpha * pv.x);
-    xv.y = static_cast<float>(xv.y + alpha * pv.y);
-    rv.x = static_cast<float>(rv.x - alpha * av.x);
-    rv.y = static_cast<float>(rv.y - alpha * av.y);
-    x[e] = xv;
-    r[e] = rv;
-    s = static_cast<double>(rv.x) * rv.x + static_cast<double>(rv.y) * rv.y;
-  }
-  s = block_sum(s, red);
-  if (threadIdx.x == 0) rr_partial[blockIdx.x] = s;
-}
-
-// rho_hist[index] = sum of partials (status = 1 when index == 0).  With
-// finish, rho_hist[0 .. index] then become the residual history sqrt(rho).
-__global__ __launch_bounds__(kVecThreads) void cg_rho_kernel(const double* __restrict__ partial,
-                                                             uint32_t count, double* __restrict__ rho_hist,
-                                                             uint32_t index, int finish,
-                                                             int* __restrict__ status) {
-  __shared__ double red[kVecThreads];
-  const double s = sum_partials(partial, count, red);
-  if (threadIdx.x == 0) {
-    rho_hist[index] = s;
-    if (index == 0) *status = 1;
-  }
-  if (!finish) return;
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i <= index; i += blockDim.x) rho_hist[i] = sqrt(fmax(rho_hist[i], 0.0));
-}
-
-// [rows][cols] -> [cols][rows] complex, 32x32 tiles, optional scale.
-__global__ __launch_bounds__(256) void transpose_kernel(const float2* __restrict__ in, uint32_t rows,
-                                                        uint32_t cols, float scale,
-                                                        float2* __restrict__ out) {
-  __shared__ float2 tile[32][33];
-  const uint32_t bx = blockIdx.x * 32, by = blockIdx.y * 32;
-  for (uint32_t i = threadIdx.y; i < 32; i += 8) {
-    const uint32_t r = by + i, c = bx + threadIdx.x;
-    if (r < rows && c < cols) tile[i][threadIdx.x] = in[static_cast<size_t>(r) * cols + c];
-  }
-  __syncthreads();
-  for (uint32_t i = threadIdx.y; i < 32; i += 8) {
-    const uint32_t c = bx + i, r = by + threadIdx.x;
-    if (r < rows && c < cols) {
-      const float2 v = tile[threadIdx.x][i];
-      out[static_cast<size_t>(c) * rows + r] = make_float2(v.x * scale, v.y * scale);
-    }
-  }
-}
-
-// [coils][n0][n1] -> [n1][coils][n0]
-__global__ __launch_bounds__(256) void smaps_transpose_kernel(const float2* __restrict__ in, uint32_t coils,
-                                                              uint32_t n0, uint32_t n1,
-                                                              float2* __restrict__ out) {
-  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const uint64_t total = static_cast<uint64_t>(coils) * n0 * n1;
-  if (e >= total) return;
-  const uint32_t x1 = static_cast<uint32_t>(e % n1);
-  const uint32_t x0 = static_cast<uint32_t>((e / n1) % n0);
-  const uint32_t c = static_cast<uint32_t>(e / (static_cast<uint64_t>(n1) * n0));
-  out[(static_cast<size_t>(x1) * coils + c) * n0 + x0] = in[e];
-}
-
-__global__ __launch_bounds__(256) void apod_transpose_kernel(const float* __restrict__ in, uint32_t n0,
-                                                             uint32_t n1, float* __restrict__ out) {
-  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= static_cast<uint64_t>(n0) * n1) return;
-  const uint32_t x1 = static_cast<uint32_t>(e % n1), x0 = static_cast<uint32_t>(e / n1);
-  out[static_cast<size_t>(x1) * n0 + x0] = in[e];
-}
-
-__global__ __launch_bounds__(256) void fill_zero_kernel(float4* __restrict__ p, uint64_t n4) {
-  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e < n4) p[e] = make_float4(0.f, 0.f, 0.f, 0.f);
-}
-
-// [coils][m] -> [m][coils] * scale (FP32 path sample layout)
-__global__ __launch_bounds__(256) void samples_to_rows_kernel(const float2* __restrict__ in, uint32_t coils,
-                                                              uint32_t m, float scale,
-                                                              float2* __restrict__ out) {
-  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= static_cast<uint64_t>(coils) * m) return;
-  const uint32_t c = static_cast<uint32_t>(e % coils), row = static_cast<uint32_t>(e / coils);
-  const float2 v = in[static_cast<size_t>(c) * m + row];
-  out[e] = make_float2(v.x * scale, v.y * scale);
-}
-
-__global__ void copy_status_kernel(const int* __restrict__ src, int* __restrict__ dst) {
-  if (threadIdx.x == 0) *dst = *src;
-}
-
-}  // namespace
+#include "cg_kernels.h"
 
 struct tsn_plan {
   tsn_plan_config cfg{};
@@ -198,6 +49,7 @@ struct tsn_plan {
   float2 *tw0 = nullptr, *tw1 = nullptr, *smaps_t = nullptr, *t_fwd = nullptr, *t_inv = nullptr;
   float* apod_t = nullptr;
   uint32_t *pan_il = nullptr, *smp_il = nullptr;  // interleaved hi | res << 16 panels
+  float2* y_rows = nullptr;                         // measurement rows [M][coils] (data consistency)
   float2 *pan32 = nullptr, *smp32 = nullptr, *grid = nullptr;
   float2 *x = nullptr, *r = nullptr, *p = nullptr, *p2 = nullptr, *ap = nullptr, *b = nullptr, *tmp = nullptr;
   double *partial = nullptr, *rr_partial = nullptr;  // <p, Ap> per image column; <r, r> per vector block
@@ -218,15 +70,6 @@ int plan_alloc(tsn_plan* pl, T** ptr, size_t count) {
   return TSN_OK;
 }
 
-int zero_async(void* p, size_t bytes, hipStream_t s) {
-  // a fill kernel (not hipMemsetAsync) so the work can sit inside a captured graph
-  if (bytes % 16) return fail(TSN_ERR_ARGUMENT, "zero fill needs a multiple of 16 bytes");
-  const uint64_t n4 = bytes / 16;
-  if (n4 == 0) return TSN_OK;
-  hipLaunchKernelGGL(fill_zero_kernel, dim3(div_up(n4, 256)), dim3(256), 0, s, static_cast<float4*>(p), n4);
-  TSN_HIP(hipGetLastError());
-  return TSN_OK;
-}
 
 FftArgs fft_args(const tsn_plan* pl) {
   FftArgs a{};
@@ -235,6 +78,7 @@ FftArgs fft_args(const tsn_plan* pl) {
   a.k0 = pl->cfg.grid0;
   a.k1 = pl->cfg.grid1;
   a.coils = pl->cfg.coils;
+  a.cstride = pl->cfg.coils;
   a.tw0 = pl->tw0;
   a.tw1 = pl->tw1;
   a.smaps_t = pl->smaps_t;
@@ -347,9 +191,9 @@ int enqueue_cg(tsn_plan* pl, const float2* measurement, uint32_t iterations, flo
   TSN_CHECK(op_adjoint_grid(pl, s));
   TSN_CHECK(op_adjoint_image(pl, pl->b, nullptr, 0.0f, inv_scale_sq, nullptr, s));
   hipLaunchKernelGGL(cg_init_kernel, dim3(pl->vec_blocks), dim3(kVecThreads), 0, s, pl->b, pl->x, pl->r,
-                     pl->p, n, pl->rr_partial);
+                     pl->p, n, pl->rr_partial, uint64_t{0}, 0u);
   hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kVecThreads), 0, s, pl->rr_partial, pl->vec_blocks, history,
-                     0u, 0, pl->status);
+                     0u, 0, pl->status, 0u, 0u);
   // rho_k lives in history[k] until the end.  Iteration k: the first forward
   // FFT pass forms p_k = r + beta p_{k-1} (k > 0, ping-pong buffers) and
   // rho_k; the last inverse pass forms <p_k, A p_k>; one kernel applies alpha.
@@ -374,10 +218,11 @@ int enqueue_cg(tsn_plan* pl, const float2* measurement, uint32_t iterations, flo
     TSN_CHECK(op_adjoint_grid(pl, s));
     TSN_CHECK(op_adjoint_image(pl, pl->ap, p_cur, scaled_lambda, inv_scale_sq, pl->partial, s));
     hipLaunchKernelGGL(cg_update_xr_kernel, dim3(pl->vec_blocks), dim3(kVecThreads), 0, s, pl->x, pl->r,
-                       p_cur, pl->ap, pl->partial, pl->cfg.image1, history, it, pl->status, n, pl->rr_partial);
+                       p_cur, pl->ap, pl->partial, pl->cfg.image1, history, it, pl->status, n, pl->rr_partial,
+                       uint64_t{0}, 0u, 0u, 0u);
   }
   hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kVecThreads), 0, s, pl->rr_partial, pl->vec_blocks, history,
-                     iterations, 1, pl->status);
+                     iterations, 1, pl->status, 0u, 0u);
   TSN_CHECK(transpose_image(pl->x, pl->cfg.image1, pl->cfg.image0, 1.0f, x_out, s));
   hipLaunchKernelGGL(copy_status_kernel, dim3(1), dim3(64), 0, s, pl->status, status_out);
   TSN_HIP(hipGetLastError());
@@ -394,14 +239,6 @@ int check_cg_args(tsn_plan* pl, const void* measurement, uint32_t iterations, fl
   return TSN_OK;
 }
 
-std::vector<float2> twiddles(uint32_t k) {
-  std::vector<float2> t(k);
-  for (uint32_t i = 0; i < k; ++i) {
-    const double ang = -2.0 * M_PI * static_cast<double>(i) / static_cast<double>(k);
-    t[i] = make_float2(static_cast<float>(std::cos(ang)), static_cast<float>(std::sin(ang)));
-  }
-  return t;
-}
 
 void destroy(tsn_plan* pl) {
   if (!pl) return;
@@ -468,6 +305,7 @@ int tsn_plan_create(const tsn_plan_config* cfg, const tsn_direction* fwd, const 
   } else {
     A(&pl->pan_il, static_cast<size_t>(pl->blocks) * grid * kN);
     A(&pl->smp_il, static_cast<size_t>(pl->blocks) * pl->m * kN);
+    A(&pl->y_rows, static_cast<size_t>(pl->m) * c.coils);
   }
   for (float2** v : {&pl->x, &pl->r, &pl->p, &pl->p2, &pl->ap, &pl->b, &pl->tmp}) A(v, px);
   A(&pl->partial, std::max<size_t>(pl->vec_blocks, c.image1));
@@ -482,7 +320,7 @@ int tsn_plan_create(const tsn_plan_config* cfg, const tsn_direction* fwd, const 
   if (rc == TSN_OK) {
     const uint64_t ns = static_cast<uint64_t>(c.coils) * px;
     hipLaunchKernelGGL(smaps_transpose_kernel, dim3(div_up(ns, 256)), dim3(256), 0, 0,
-                       static_cast<const float2*>(smaps), c.coils, c.image0, c.image1, pl->smaps_t);
+                       static_cast<const float2*>(smaps), c.coils, c.image0, c.image1, c.coils, pl->smaps_t);
     hipLaunchKernelGGL(apod_transpose_kernel, dim3(div_up(px, 256)), dim3(256), 0, 0,
                        static_cast<const float*>(apodization), c.image0, c.image1, pl->apod_t);
     // the adjoint grid's inactive rows are never written: zero it once here
@@ -527,6 +365,40 @@ int tsn_normal(tsn_plan* pl, const void* image, void* out, float inv_scale_sq, f
   hipStream_t s = static_cast<hipStream_t>(stream);
   TSN_CHECK(transpose_image(static_cast<const float2*>(image), pl->cfg.image0, pl->cfg.image1, 1.0f, pl->tmp, s));
   TSN_CHECK(op_normal(pl, pl->tmp, pl->ap, inv_scale_sq, scaled_lambda, nullptr, s));
+  return transpose_image(pl->ap, pl->cfg.image1, pl->cfg.image0, 1.0f, static_cast<float2*>(out), s);
+}
+
+int tsn_data_consistency(tsn_plan* pl, const void* image, const void* measurement, const float* density,
+                         float measurement_scale, void* out, void* stream) {
+  if (!pl || !image || !measurement || !out) return fail(TSN_ERR_ARGUMENT, "null argument");
+  if (!std::isfinite(measurement_scale)) return fail(TSN_ERR_ARGUMENT, "measurement scale must be finite");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint32_t coils = pl->cfg.coils;
+  DcTerms dc;
+  dc.density = density;
+  dc.scale = measurement_scale;
+  TSN_CHECK(transpose_image(static_cast<const float2*>(image), pl->cfg.image0, pl->cfg.image1, 1.0f, pl->tmp, s));
+  TSN_CHECK(op_forward_grid(pl, pl->tmp, s));
+  if (is_fp32(pl)) {
+    // CSR forward into the FP32 sample rows, then the residual in place (y coil-major)
+    dc.y = static_cast<const float*>(measurement);
+    dc.row_stride = 2;
+    dc.coil_stride = 2 * pl->m;
+    TSN_CHECK(launch_interp_csr(pl->fwd, pl->pan32, coils, 1, false, false, grid_norm(pl), pl->smp32, s));
+    TSN_CHECK(launch_dc_residual_rows(pl->smp32, pl->m, coils, dc, s));
+  } else {
+    // y to rows (one 64-byte read per sample row in the epilogue), residual fused
+    const uint64_t n = static_cast<uint64_t>(coils) * pl->m;
+    hipLaunchKernelGGL(samples_to_rows_kernel, dim3(div_up(n, 256)), dim3(256), 0, s,
+                       static_cast<const float2*>(measurement), coils, pl->m, 1.0f, pl->y_rows);
+    TSN_HIP(hipGetLastError());
+    dc.y = reinterpret_cast<const float*>(pl->y_rows);
+    dc.row_stride = 2 * coils;
+    dc.coil_stride = 2;
+    TSN_CHECK(launch_interp_dc(pl->fwd, interp_kind(pl), pl->pan_il, coils, grid_norm(pl), dc, pl->smp_il, s));
+  }
+  TSN_CHECK(op_adjoint_grid(pl, s));
+  TSN_CHECK(op_adjoint_image(pl, pl->ap, nullptr, 0.0f, 1.0f, nullptr, s));
   return transpose_image(pl->ap, pl->cfg.image1, pl->cfg.image0, 1.0f, static_cast<float2*>(out), s);
 }
 

@@ -96,6 +96,53 @@ struct WorkItem {
   int32_t tile_begin;
   int32_t tile_end;
 };
+// One coil block of one problem of a batched interpolation launch.
+struct InterpBatchPart {
+  const tsn_direction* dir;
+  const uint32_t* panel_il;  // the block's interleaved input panel [n_cols][16]
+  float* out_f32;            // the problem's float2 output (row- or coil-major)
+  uint32_t* out_il;          // the block's interleaved split output panel [n_out][16]
+  uint32_t coils;            // coils of the problem
+  uint32_t coil_base;        // first coil of the block
+};
+
+// Device tables of a batched launch: parts sharing a direction are merged
+// unit-major (consecutive waves of a block take the same unit of different
+// parts, so its tiles are read from HBM once and then from cache).
+struct InterpBatch {
+  void* problems = nullptr;
+  WorkItem* items = nullptr;
+  uint32_t* item_problem = nullptr;
+  WorkItem* long_items = nullptr;
+  uint32_t* long_problem = nullptr;
+  uint32_t n_problems = 0, n_items = 0, n_long = 0;
+  int values = TSN_VALUES_FP16;
+  bool dense = false;
+};
+int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* out);
+void interp_batch_free(InterpBatch* b);
+// kind: 0 split FP16, 1 dense control, 2 FP16 hi only, 3 split BF16; out_mode:
+// 1 row-major float2, 3 interleaved split (FP16 or BF16 as the kind).
+int launch_interp_batch(const InterpBatch& b, int kind, int out_mode, bool conj, float scale, hipStream_t s);
+
+// Data-consistency terms: residual (prediction - scale * y) * w per sample
+// row, y read at row * row_stride + coil * coil_stride + part (floats).
+struct DcTerms {
+  const float* y = nullptr;
+  const float* density = nullptr;  // [rows] or null (all ones)
+  float scale = 1.0f;
+  uint32_t row_stride = 0, coil_stride = 0;
+};
+// Forward interpolation of an interleaved split panel with the residual fused
+// into its epilogue; writes the interleaved split residual panel.
+int launch_interp_dc(const tsn_direction* d, int kind, const uint32_t* panel, uint32_t coils, float scale,
+                     const DcTerms& dc, uint32_t* out_il, hipStream_t s);
+// The same residual from stored predictions float2 [rows][coils].
+int launch_dc_residual(const float2* pred, uint32_t rows, uint32_t coils, const DcTerms& dc, bool bf16,
+                       uint32_t* out_il, hipStream_t s);
+// In place on FP32 rows float2 [rows][coils] (FP32 plans): pred <- residual.
+int launch_dc_residual_rows(float2* rows_io, uint32_t rows, uint32_t coils, const DcTerms& dc, hipStream_t s);
+
 }  // namespace tsn
 
 struct tsn_host_pack : tsn::HostPack {};
@@ -125,6 +172,7 @@ struct tsn_direction {
   uint32_t n_items = 0, n_long = 0;
   uint32_t unit_tiles = 0;           // tile budget of one regular unit
   std::vector<uint32_t> host_offsets;  // group_offsets kept for rescheduling
+  std::vector<tsn::WorkItem> host_items, host_long;  // the current schedule (batched launches merge it)
   std::vector<void*> retired;          // earlier schedules (graphs may reference them)
   // ---- FP32 CSR (device)
   uint32_t csr_rows = 0;

@@ -85,6 +85,14 @@ class Direction:
         """Tile budget of one regular work unit (0 for CSR directions)."""
         return int(lib().tsn_direction_unit_tiles(self._h))
 
+    def autotune(self, coils: int = 8) -> tuple[int, float]:
+        """Time the interpolation over the candidate unit sizes and keep the
+        fastest; returns (unit_tiles, microseconds per launch)."""
+
+        best, us = C.c_uint32(), C.c_float()
+        check(lib().tsn_direction_autotune(self._h, int(coils), _stream(), C.byref(best), C.byref(us)))
+        return int(best.value), float(us.value)
+
     def schedule(self, unit_tiles: int = 0) -> None:
         """Re-cut the work units (runs of whole groups of at most unit_tiles
         tiles; 0 = default).  Results are bitwise independent of the cut."""
@@ -174,6 +182,42 @@ def interp_il(direction: Direction, panel: torch.Tensor, out: torch.Tensor, coil
     _require(out, torch.complex64, coils * direction.rows, "out")
     check(lib().tsn_interp_il(direction.handle, _ptr(panel), _ptr(out), coils, int(conjugate), int(accumulate),
                               float(scale), LAYOUTS[layout], _stream()))
+    return out
+
+
+DC_MODES = {"materialized": _lib.DC_MATERIALIZED, "fused": _lib.DC_FUSED}
+
+
+def interp_dc(direction: Direction, panel: torch.Tensor, measurements: torch.Tensor, coils: int = 8,
+              density: torch.Tensor | None = None, scale: float = 1.0, measurement_scale: float = 1.0,
+              mode: str = "fused", layout: str = "coil_major", workspace: torch.Tensor | None = None,
+              out: torch.Tensor | None = None) -> torch.Tensor:
+    """Data-consistency residual panel of the forward direction (the adjoint's input):
+    split((scale * G panel - measurement_scale * y) * density), int32 [blocks, rows, 16].
+
+    measurements: complex64 [coils, rows] (``layout="coil_major"``, the reference's)
+    or [rows, coils] (``"row_major"``); density float32 [rows] or None (ones).
+    ``mode="materialized"`` interpolates into ``workspace`` (complex64
+    [rows, coils], allocated when None) and forms the residual in a second pass;
+    ``"fused"`` forms it in the interpolation epilogue.  Both give the same bits
+    (reference ``JKF_DATA_CONSISTENCY_MODE`` 1 and 2)."""
+
+    rows = direction.rows
+    _require(panel, torch.int32, coil_blocks(coils) * direction.cols * 16, "panel")
+    _require(measurements, torch.complex64, coils * rows, "measurements")
+    if density is not None:
+        _require(density, torch.float32, rows, "density")
+    if mode == "materialized" and workspace is None:
+        workspace = torch.empty((rows, coils), dtype=torch.complex64, device=panel.device)
+    if workspace is not None:
+        _require(workspace, torch.complex64, coils * rows, "workspace")
+    if out is None:
+        out = torch.empty((coil_blocks(coils), rows, 16), dtype=torch.int32, device=panel.device)
+    _require(out, torch.int32, coil_blocks(coils) * rows * 16, "out")
+    check(lib().tsn_interp_dc(direction.handle, _ptr(panel), _ptr(measurements), LAYOUTS[layout],
+                              _ptr(density) if density is not None else None, coils, float(scale),
+                              float(measurement_scale), DC_MODES[mode],
+                              _ptr(workspace) if workspace is not None else None, _ptr(out), _stream()))
     return out
 
 
