@@ -141,7 +141,8 @@ int tsn_direction_upload_csr(const tsn_kb_matrix* kb, int view, tsn_direction** 
 int tsn_direction_header(const tsn_direction* dir, tsn_pack_header* out);
 /* Re-cut the work schedule of a packed direction: regular units are runs of
  * whole groups holding at most unit_tiles tiles each (0 = default: 1.5 rounds
- * of resident waves, 2..8 tiles).  Earlier schedules stay valid for graphs
+ * of resident waves, 2..8 tiles; larger values act as 16, the most a wave
+ * stages in LDS).  Earlier schedules stay valid for graphs
  * captured before the call.  Synchronizes the device; no counterpart in the
  * reference (its launch maps one group per warp), used by the autotuner. */
 int tsn_direction_schedule(tsn_direction* dir, uint32_t unit_tiles);

@@ -182,7 +182,7 @@ def test_schedule_cut_is_bitwise_invariant(view):
     outs = []
     for unit in (0, 1, 3, 64):
         d.schedule(unit)
-        assert unit == 0 or d.unit_tiles == unit
+        assert unit == 0 or d.unit_tiles == min(unit, 16)  # units hold at most 16 tiles
         out = torch.zeros((12, header["output_size"]), dtype=torch.complex64, device="cuda")
         device.interp(d, h, r, out, coils=12, conjugate=view == "G_T")
         outs.append(out.cpu().numpy().view(np.uint32))

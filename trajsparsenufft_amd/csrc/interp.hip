@@ -257,6 +257,35 @@ __device__ __forceinline__ TileCols fetch_cols(const TileRsrc<NB>& r, int t, int
   return c;
 }
 
+// A regular unit's column words (at most kSplitAboveTiles tiles, 128 bytes
+// each) are copied once into the wave's LDS slice, so the gathers of a tile
+// wait for an LDS read instead of a global load issued one step earlier (the
+// chain column word -> gather address -> gather otherwise costs a full memory
+// latency per tile).  Words past the unit read neighbours or zeros (unused).
+constexpr int kStageWords = kSplitAboveTiles * kK;  // per wave
+
+template <int NB>
+__device__ __forceinline__ void stage_cols(const TileRsrc<NB>& r, int first, int n, uint32_t* stage, int lane) {
+  first = __builtin_amdgcn_readfirstlane(first);
+  const u32x4_t v0 = __builtin_amdgcn_raw_buffer_load_b128(r.cols, lane * 16, first * (kK * 4), 0);
+  reinterpret_cast<u32x4_t*>(stage)[lane] = v0;
+  if (__builtin_amdgcn_readfirstlane(n) > 8) {
+    const u32x4_t v1 = __builtin_amdgcn_raw_buffer_load_b128(r.cols, 1024 + lane * 16, first * (kK * 4), 0);
+    reinterpret_cast<u32x4_t*>(stage)[64 + lane] = v1;
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Column words of the unit's k-th tile from the stage (the 16 lanes of a
+// quarter read the same 32 bytes: LDS broadcast).
+__device__ __forceinline__ TileCols staged_cols(const uint32_t* stage, int k, int lane) {
+  const u32x4_t* p = reinterpret_cast<const u32x4_t*>(stage + k * kK) + (lane >> 4) * 2;
+  TileCols c;
+  c.c0 = p[0];
+  c.c1 = p[1];
+  return c;
+}
+
 // Stage 2 of a tile: A fragment, sparse index bytes, last-tile flag and the
 // raw gathered panel words (kept unpacked until the multiply so no wait is
 // forced before then).  IL: one dword (hi | res << 16) per row; otherwise one
@@ -377,7 +406,8 @@ struct PipeDepth {
 template <int KIND, int NB, bool IL, bool GROUPS, int OM, bool CONJ, bool ACC>
 __device__ __forceinline__ void stream_tiles(const InterpArgs& a, const TileRsrc<NB>& r, int first, int end,
                                              int stride, int group, int group_end,
-                                             float4_t (&acc_re)[NB], float4_t (&acc_im)[NB], int lane) {
+                                             float4_t (&acc_re)[NB], float4_t (&acc_im)[NB], int lane,
+                                             const uint32_t* stage = nullptr) {
   constexpr int D = PipeDepth<KIND, NB, IL>::value;
   constexpr int U = D + 1 > 2 ? D + 1 : 2;
   // wave-uniform bounds: scalar control flow and scalar tile offsets
@@ -389,6 +419,11 @@ __device__ __forceinline__ void stream_tiles(const InterpArgs& a, const TileRsrc
   const int n = (end - first + stride - 1) / stride;
   const int last = first + (n - 1) * stride;
   auto tile = [&](int i) { return min(first + i * stride, last); };
+  // GROUPS units (stride 1) read their column words from the LDS stage
+  auto cols_of = [&](int i) {
+    if constexpr (GROUPS) return staged_cols(stage, tile(i) - first, lane);
+    else return fetch_cols(r, tile(i), lane);
+  };
   int4 rows = make_int4(-1, -1, -1, -1);
   if constexpr (GROUPS) rows = load_rows(r, g, lane);
   TileCols cols[U];
@@ -396,7 +431,7 @@ __device__ __forceinline__ void stream_tiles(const InterpArgs& a, const TileRsrc
   // The scheduler must keep the issue order: the wait counter is in order, and
   // the loop's waits are computed over both the prologue and the back edge.
 #pragma unroll
-  for (int i = 0; i <= D; ++i) cols[i % U] = fetch_cols(r, tile(i), lane);
+  for (int i = 0; i <= D; ++i) cols[i % U] = cols_of(i);
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int i = 0; i < D; ++i) {
@@ -410,7 +445,7 @@ __device__ __forceinline__ void stream_tiles(const InterpArgs& a, const TileRsrc
       __builtin_amdgcn_sched_barrier(0);
       // loads run unconditionally (clamped), only the multiply is skipped past
       // the end, so no loaded register is merged across a branch
-      cols[(j + D + 1) % U] = fetch_cols(r, tile(i + D + 1), lane);
+      cols[(j + D + 1) % U] = cols_of(i + D + 1);
       __builtin_amdgcn_sched_barrier(0);
       fetch_ops<KIND, NB, IL>(r, tile(i + D), cols[(j + D) % U], lane, ops[(j + D) % U]);
       if (i < n) {
@@ -465,7 +500,7 @@ __device__ __forceinline__ void long_group(const InterpArgs& a, const WorkItem& 
 // A regular unit: one wave streams its whole groups.
 template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC>
 __device__ __forceinline__ void regular_unit(const InterpArgs& a, const WorkItem& u, float4_t (&acc_re)[NB],
-                                             float4_t (&acc_im)[NB], int lane) {
+                                             float4_t (&acc_im)[NB], int lane, uint32_t* stage) {
   if (u.tile_begin == u.tile_end) {  // empty group: zero rows
     const int4 rows = group_rows_of(a, static_cast<uint32_t>(u.group), lane);
 #pragma unroll
@@ -473,8 +508,9 @@ __device__ __forceinline__ void regular_unit(const InterpArgs& a, const WorkItem
     return;
   }
   const TileRsrc<NB> r = make_tile_rsrc<NB, IL>(a);
+  stage_cols(r, u.tile_begin, u.tile_end - u.tile_begin, stage, lane);
   stream_tiles<KIND, NB, IL, true, OM, CONJ, ACC>(a, r, u.tile_begin, u.tile_end, 1, u.group, u.group_end,
-                                                  acc_re, acc_im, lane);
+                                                  acc_re, acc_im, lane, stage);
 }
 
 // Blocks of kTileWaves waves.  The first n_long blocks each own one long group
@@ -483,7 +519,9 @@ __device__ __forceinline__ void regular_unit(const InterpArgs& a, const WorkItem
 // of one XCD contiguous in pack order.
 template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC>
 __global__ __launch_bounds__(kTileWaves * 64) void interp_tiles_kernel(InterpArgs a) {
+  // long-group blocks reduce through it; regular blocks stage column words in it
   __shared__ float4_t red[kTileWaves][NB][2][64];
+  static_assert(sizeof(red) >= sizeof(uint32_t) * kStageWords * kTileWaves, "LDS stage does not fit");
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   float4_t acc_re[NB], acc_im[NB];
@@ -509,14 +547,15 @@ __global__ __launch_bounds__(kTileWaves * 64) void interp_tiles_kernel(InterpArg
   const uint32_t item_id = xcd_remap(rb, gridDim.x - a.n_long) * kTileWaves + wave;
   if (item_id >= a.n_items) return;
   const WorkItem u = a.items[item_id];
+  uint32_t* stage = reinterpret_cast<uint32_t*>(&red[0][0][0][0]) + wave * kStageWords;
   if constexpr (NB == 1 && IL) {
     if (a.problems) {
       regular_unit<KIND, NB, IL, OM, CONJ, ACC>(problem_args(a, a.item_problem[item_id]), u, acc_re, acc_im,
-                                                lane);
+                                                lane, stage);
       return;
     }
   }
-  regular_unit<KIND, NB, IL, OM, CONJ, ACC>(a, u, acc_re, acc_im, lane);
+  regular_unit<KIND, NB, IL, OM, CONJ, ACC>(a, u, acc_re, acc_im, lane, stage);
 }
 
 
@@ -994,6 +1033,8 @@ static int schedule_units(tsn_direction* d, uint32_t unit_tiles) {
     const uint64_t waves = static_cast<uint64_t>(cus) * kTileWaves * 3 / 2;
     target = std::clamp<int64_t>(static_cast<int64_t>((regular_tiles + waves - 1) / waves), 2, 8);
   }
+  // a unit's column words are staged in the wave's LDS slice: at most 16 tiles
+  target = std::min<int64_t>(target, kSplitAboveTiles);
   std::vector<WorkItem> items, long_items;
   WorkItem cur{-1, -1, 0, 0};
   auto flush = [&]() {
