@@ -156,11 +156,13 @@ def measured_traffic(kernel_substr: str):
 
 
 def interp_roofline(torch, packs, steps, warmup, log):
-    """Standalone launches of the packed interpolation (interleaved split
-    panel, the layout the reconstruction uses), timed with events on the
-    launch stream; algorithmic bytes per launch as defined for the unit of one
-    complex interpolation application: nnz * (2 + 2 + 4) + 2 * n_in * 2C * 2 +
-    n_out * C * 8."""
+    """Standalone launches of the packed interpolation exactly as the
+    reconstruction runs it (interleaved split panel in; the forward writes the
+    adjoint's split panel, the adjoint writes row-major grid rows), timed with
+    events on the launch stream; algorithmic bytes per launch as defined for
+    the unit of one complex interpolation application: nnz * (2 + 2 + 4) +
+    2 * n_in * 2C * 2 + n_out * C * 8 (the split panel out is the same
+    n_out * 16 channels * 4 bytes)."""
 
     from trajsparsenufft_amd import device
     from trajsparsenufft_amd.device import Direction
@@ -174,15 +176,16 @@ def interp_roofline(torch, packs, steps, warmup, log):
         return device.split_panel_il(torch.from_numpy(v.astype(np.complex64)).cuda(), COILS)
 
     gp, sp = panel(g), panel(m)
-    out_s = torch.empty((COILS, m), dtype=torch.complex64, device="cuda")
-    out_g = torch.zeros((COILS, g), dtype=torch.complex64, device="cuda")
-    fwd_ms = time_launches(torch, lambda: device.interp_il(fwd, gp, out_s, COILS, False, False, 1.0 / GRID),
+    out_s = torch.empty((1, m, 16), dtype=torch.int32, device="cuda")
+    out_g = torch.zeros((g, COILS), dtype=torch.complex64, device="cuda")
+    fwd_ms = time_launches(torch, lambda: device.interp_il(fwd, gp, out_s, COILS, False, False, 1.0 / GRID,
+                                                           "split_il"), steps, warmup)
+    adj_ms = time_launches(torch, lambda: device.interp_il(adj, sp, out_g, COILS, True, False, 1.0, "row_major"),
                            steps, warmup)
-    adj_ms = time_launches(torch, lambda: device.interp_il(adj, sp, out_g, COILS, True), steps, warmup)
     res = {}
-    for name, ms, n_in, n_out, conj in (("forward", fwd_ms, g, m, "false"), ("adjoint", adj_ms, m, g, "true")):
+    for name, ms, n_in, n_out, conj, om in (("forward", fwd_ms, g, m, "false", 3), ("adjoint", adj_ms, m, g, "true", 1)):
         b = nnz * 8 + 2 * n_in * 2 * COILS * 2 + n_out * COILS * 8
-        kernel = f"interp_tiles_kernel<0, 1, true, 0, {conj}, false>"
+        kernel = f"interp_tiles_kernel<0, 1, true, {om}, {conj}, false>"
         res[name] = {"kernel": kernel, "us": ms * 1e3, "bytes": b, "GBs": b / (ms * 1e-3) / 1e9,
                      "traffic": measured_traffic(kernel)}
         log(f"[bench] interp {name} {ms * 1e3:.1f} us ({res[name]['GBs']:.0f} GB/s algorithmic, "

@@ -47,7 +47,8 @@ enum {
 /* Output layouts of the interpolation entry points. */
 enum {
   TSN_OUT_COIL_MAJOR = 0, /* float2 [coil][row]   (reference output layout) */
-  TSN_OUT_ROW_MAJOR = 1   /* float2 [row][coil]   (coil-interleaved rows)  */
+  TSN_OUT_ROW_MAJOR = 1,  /* float2 [row][coil]   (coil-interleaved rows)  */
+  TSN_OUT_SPLIT_IL = 2    /* interleaved split panel of the output rows (tsn_interp_il only) */
 };
 
 /* Pack header, 52 bytes little endian, struct "<Q11I"
@@ -140,8 +141,8 @@ int tsn_direction_load(const char* real_dir, const char* imag_dir, tsn_direction
 int tsn_direction_upload_csr(const tsn_kb_matrix* kb, int view, tsn_direction** out);
 int tsn_direction_header(const tsn_direction* dir, tsn_pack_header* out);
 /* Re-cut the work schedule of a packed direction: regular units are runs of
- * whole groups holding at most unit_tiles tiles each (0 = default: 1.5 rounds
- * of resident waves, 2..8 tiles; larger values act as 16, the most a wave
+ * whole groups holding at most unit_tiles tiles each (0 = default: the tiles
+ * over 12 waves per CU, 4..14 tiles; larger values act as 16, the most a wave
  * stages in LDS).  Earlier schedules stay valid for graphs
  * captured before the call.  Synchronizes the device; no counterpart in the
  * reference (its launch maps one group per warp), used by the autotuner. */
@@ -186,7 +187,10 @@ int tsn_split_panel(const void* in, uint32_t rows, uint32_t coils, float scale, 
  * halves (TSN_VALUES_FP16, rounded exactly as tsn_split_panel) or bfloat16
  * halves (TSN_VALUES_BF16, round to nearest even).  tsn_interp_il is
  * tsn_interp on such a panel with the direction's own value format (same
- * tiles, same sums, same output for FP16). */
+ * tiles, same sums, same output for FP16).  With out_layout TSN_OUT_SPLIT_IL
+ * (accumulate 0) it writes its result as such a panel of its output rows
+ * (int32 [blocks][rows][16], the direction's value format): the forward
+ * interpolation handing the adjoint its input, as inside a plan. */
 int tsn_split_panel_il(const void* in, uint32_t rows, uint32_t coils, float scale, int values,
                        void* panel, void* stream);
 int tsn_interp_il(const tsn_direction* dir, const void* panel, void* out, uint32_t coils,

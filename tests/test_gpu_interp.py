@@ -214,6 +214,12 @@ def test_interleaved_panel_path_is_bitwise_the_split_path(view, coils):
     assert np.array_equal(out.cpu().numpy().view(np.uint32), out_il.cpu().numpy().view(np.uint32))
     device.interp_il(d, il, out_il, coils=coils, conjugate=conj, scale=0.25, accumulate=True)
     np.testing.assert_allclose(out_il.cpu().numpy(), 2 * out.cpu().numpy(), rtol=1e-6, atol=1e-6)
+    # split-panel output (the plan's forward): the split of the float2 result, bit for bit
+    blocks = (coils + 7) // 8
+    out_sp = torch.zeros((blocks, header["output_size"], 16), dtype=torch.int32, device="cuda")
+    device.interp_il(d, il, out_sp, coils=coils, conjugate=conj, scale=0.25, layout="split_il")
+    ref_sp = device.split_panel_il(out.contiguous(), coils)
+    assert np.array_equal(out_sp.cpu().numpy(), ref_sp.cpu().numpy())
 
 
 @pytest.mark.parametrize("view", ["G", "G_T"])

@@ -15,7 +15,7 @@ LIBRARY_PATH = Path(__file__).resolve().parent / "libtsn.so"
 TSN_OK, TSN_ERR_RUNTIME, TSN_ERR_INCOMPATIBLE, TSN_ERR_CG, TSN_ERR_ARGUMENT = 0, 1, 2, 3, 4
 PREC_SPLIT16, PREC_DENSE16, PREC_FP32, PREC_HI16, PREC_SPLITBF16 = 0, 1, 2, 3, 4
 VALUES_FP16, VALUES_BF16 = 0, 1
-OUT_COIL_MAJOR, OUT_ROW_MAJOR = 0, 1
+OUT_COIL_MAJOR, OUT_ROW_MAJOR, OUT_SPLIT_IL = 0, 1, 2
 DC_MATERIALIZED, DC_FUSED = 1, 2
 
 

@@ -44,8 +44,11 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
 struct InterpArgs {
   const WorkItem* items;
   uint32_t n_items;
-  const WorkItem* long_items;
-  uint32_t n_long;        // multiple of 8 (padding entries have group < 0)
+  const ChunkItem* chunks;  // long-group chunks
+  uint32_t n_chunks;
+  const uint32_t* order;    // [n_items + n_chunks] launch order (kChunkTag marks chunks)
+  float* partials;          // their partial slots [slot][kSlotFloats]
+  uint32_t* counters;       // arrival counter per long group
   const int32_t* group_rows;
   const uint32_t* group_offsets;
   const uint32_t* tile_cols;  // column * 32 | last-tile flag (bit 31 of word 2 of each lane segment)
@@ -69,7 +72,6 @@ struct InterpArgs {
   // interleaved panels): items[i] belongs to problems[item_problem[i]].
   const struct InterpProblem* problems;
   const uint32_t* item_problem;
-  const uint32_t* long_problem;
   // Data-consistency epilogue of split outputs: each value v becomes
   // (v - dc_scale * y) * w before the split.  y is read at
   // row * dc_row_stride + coil * dc_coil_stride + part (floats); null = off.
@@ -248,20 +250,12 @@ struct TileCols {
   u32x4_t c0, c1;
 };
 
-template <int NB>
-__device__ __forceinline__ TileCols fetch_cols(const TileRsrc<NB>& r, int t, int lane) {
-  TileCols c;
-  const int off = (lane >> 4) * 32;
-  c.c0 = __builtin_amdgcn_raw_buffer_load_b128(r.cols, off, t * (kK * 4), 0);
-  c.c1 = __builtin_amdgcn_raw_buffer_load_b128(r.cols, off + 16, t * (kK * 4), 0);
-  return c;
-}
-
-// A regular unit's column words (at most kSplitAboveTiles tiles, 128 bytes
-// each) are copied once into the wave's LDS slice, so the gathers of a tile
-// wait for an LDS read instead of a global load issued one step earlier (the
-// chain column word -> gather address -> gather otherwise costs a full memory
-// latency per tile).  Words past the unit read neighbours or zeros (unused).
+// The column words of a unit or chunk (at most kSplitAboveTiles tiles, 128
+// bytes each) are copied once into the wave's LDS slice, so the gathers of a
+// tile wait for an LDS read instead of a global load issued one step earlier
+// (the chain column word -> gather address -> gather otherwise costs a full
+// memory latency per tile).  Words past the range read neighbours or zeros
+// (unused).
 constexpr int kStageWords = kSplitAboveTiles * kK;  // per wave
 
 template <int NB>
@@ -381,10 +375,10 @@ __device__ __forceinline__ void mma_tile(const TileOps<NB, IL>& o, float4_t (&ac
   }
 }
 
-// Gather stages in flight ahead of the multiply, as deep as 128 registers per
-// lane (the cap of a 16-wave block) hold without spilling: 2 for one coil
-// block of the interleaved panel, 0 for two coil blocks of separate panels or
-// of dense-control tiles, else 1.
+// Gather stages in flight ahead of the multiply: 2 for one coil block of the
+// interleaved panel, 0 for two coil blocks of separate panels or of
+// dense-control tiles, else 1 (deeper stages measured no faster and cost
+// occupancy).
 template <int KIND, int NB, bool IL>
 struct PipeDepth {
   static constexpr int value = (IL && NB == 1 && KIND != kDenseSplit) ? 2
@@ -392,22 +386,21 @@ struct PipeDepth {
                                                                           : 1;
 };
 
-// Stream tiles first, first + stride, ... (< end) of one wave through a
-// software pipeline.  Step i issues the column words of tile i + D + 1, then
-// the operand loads of tile i + D, then multiplies tile i (D = PipeDepth);
-// loads return in issue order, so the wait for the words of tile i + D leaves
-// the operand loads of D tiles in flight.  Tile i lives in register slot
-// i % U of both rings (U = max(D + 1, 2), the loop unrolled U times so every
-// slot index is a constant).  Indices past the end are clamped to the last
-// tile (valid, unused loads).  GROUPS: the tiles are those of the consecutive
-// groups group, group + 1, ... (< group_end); the last tile of each runs the
-// epilogue with the group's rows (loaded one group ahead) and resets the
-// accumulators.  Otherwise everything accumulates.
+// Stream tiles first .. end - 1 (at most kSplitAboveTiles, column words
+// staged) of one wave through a software pipeline.  Step i reads the column
+// words of tile i + D + 1 from the stage, issues the operand loads of tile
+// i + D, then multiplies tile i (D = PipeDepth); operand loads return in
+// issue order, so the multiply's wait leaves D tiles of loads in flight.
+// Tile i lives in register slot i % U of both rings (U = max(D + 1, 2), the
+// loop unrolled U times so every slot index is a constant).  Indices past the
+// end are clamped to the last tile (valid, unused loads).  GROUPS: the tiles
+// are those of the consecutive groups group, group + 1, ... (< group_end); the
+// last tile of each runs the epilogue with the group's rows (loaded one group
+// ahead) and resets the accumulators.  Otherwise everything accumulates.
 template <int KIND, int NB, bool IL, bool GROUPS, int OM, bool CONJ, bool ACC>
 __device__ __forceinline__ void stream_tiles(const InterpArgs& a, const TileRsrc<NB>& r, int first, int end,
-                                             int stride, int group, int group_end,
-                                             float4_t (&acc_re)[NB], float4_t (&acc_im)[NB], int lane,
-                                             const uint32_t* stage = nullptr) {
+                                             int group, int group_end, float4_t (&acc_re)[NB],
+                                             float4_t (&acc_im)[NB], int lane, const uint32_t* stage) {
   constexpr int D = PipeDepth<KIND, NB, IL>::value;
   constexpr int U = D + 1 > 2 ? D + 1 : 2;
   // wave-uniform bounds: scalar control flow and scalar tile offsets
@@ -416,14 +409,10 @@ __device__ __forceinline__ void stream_tiles(const InterpArgs& a, const TileRsrc
   int g = __builtin_amdgcn_readfirstlane(group);
   const int g_last = __builtin_amdgcn_readfirstlane(group_end) - 1;
   if (first >= end) return;
-  const int n = (end - first + stride - 1) / stride;
-  const int last = first + (n - 1) * stride;
-  auto tile = [&](int i) { return min(first + i * stride, last); };
-  // GROUPS units (stride 1) read their column words from the LDS stage
-  auto cols_of = [&](int i) {
-    if constexpr (GROUPS) return staged_cols(stage, tile(i) - first, lane);
-    else return fetch_cols(r, tile(i), lane);
-  };
+  const int n = end - first;
+  const int last = end - 1;
+  auto tile = [&](int i) { return min(first + i, last); };
+  auto cols_of = [&](int i) { return staged_cols(stage, tile(i) - first, lane); };
   int4 rows = make_int4(-1, -1, -1, -1);
   if constexpr (GROUPS) rows = load_rows(r, g, lane);
   TileCols cols[U];
@@ -467,33 +456,64 @@ __device__ __forceinline__ void stream_tiles(const InterpArgs& a, const TileRsrc
   }
 }
 
-// A long group: the block's waves stride over its tiles; partial sums are
-// added through LDS in wave order (deterministic), wave 0 runs the epilogue.
+// One chunk of a long group: stream its tiles, publish the partial sums, and
+// if this wave is the group's last arrival, add every chunk's slot in chunk
+// order and write the group's rows.  Slots are written and read with
+// agent-scope (device-coherent) relaxed atomic stores and loads, which reach
+// past the per-XCD L2 without a cache flush; the stores complete (vmcnt 0)
+// before the arrival is counted, and the slots are read only after the count
+// shows every chunk done.  A full agent-scope fence here would write back the
+// whole L2 per chunk (measured: the launch went from 29 to 230 us).  The
+// last wave resets the counter for the next launch.  No wave waits on another.
 template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC>
-__device__ __forceinline__ void long_group(const InterpArgs& a, const WorkItem& it,
-                                           float4_t (&red)[kTileWaves][NB][2][64], float4_t (&acc_re)[NB],
-                                           float4_t (&acc_im)[NB], int lane, int wave) {
-  {
-    const TileRsrc<NB> r = make_tile_rsrc<NB, IL>(a);
-    stream_tiles<KIND, NB, IL, false, OM, CONJ, ACC>(a, r, it.tile_begin + wave, it.tile_end, kTileWaves,
-                                                     it.group, it.group_end, acc_re, acc_im, lane);
+__device__ __forceinline__ void long_chunk(const InterpArgs& a, const ChunkItem& c, float4_t (&acc_re)[NB],
+                                           float4_t (&acc_im)[NB], int lane, uint32_t* stage) {
+  const TileRsrc<NB> r = make_tile_rsrc<NB, IL>(a);
+  stage_cols(r, c.tile_begin, c.tile_end - c.tile_begin, stage, lane);
+  stream_tiles<KIND, NB, IL, false, OM, CONJ, ACC>(a, r, c.tile_begin, c.tile_end, c.group, c.group + 1, acc_re,
+                                                   acc_im, lane, stage);
+  // slot layout: [coil block][re, im][lane][4] floats
+  float* mine = a.partials + static_cast<size_t>(c.slot0 + c.chunk) * kSlotFloats;
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      red[wave][b][0][lane] = acc_re[b];
-      red[wave][b][1][lane] = acc_im[b];
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      __hip_atomic_store(mine + ((b * 2 + 0) * 64 + lane) * 4 + k, acc_re[b][k], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(mine + ((b * 2 + 1) * 64 + lane) * 4 + k, acc_im[b][k], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
     }
-    __syncthreads();
-    if (wave != 0) return;
-    const int4 rows = group_rows_of(a, static_cast<uint32_t>(it.group), lane);
+  __builtin_amdgcn_s_waitcnt(0);  // every slot store acknowledged
+  asm volatile("" ::: "memory");
+  uint32_t arrived = 0;
+  if (lane == 0)
+    arrived = __hip_atomic_fetch_add(a.counters + c.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  arrived = __builtin_amdgcn_readfirstlane(arrived);
+  if (arrived + 1 != c.n_chunks) return;
+  asm volatile("" ::: "memory");
+  if (lane == 0) __hip_atomic_store(a.counters + c.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const float* first = a.partials + static_cast<size_t>(c.slot0) * kSlotFloats;
+  const int4 rows = group_rows_of(a, static_cast<uint32_t>(c.group), lane);
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      float4_t re = red[0][b][0][lane], im = red[0][b][1][lane];
-      for (int w2 = 1; w2 < kTileWaves; ++w2) {
-        re += red[w2][b][0][lane];
-        im += red[w2][b][1][lane];
+  for (int b = 0; b < NB; ++b) {
+    float4_t re, im;
+    for (uint32_t q = 0; q < c.n_chunks; ++q) {
+      const float* p = first + static_cast<size_t>(q) * kSlotFloats;
+      float4_t pr, pi;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        pr[k] = __hip_atomic_load(p + ((b * 2 + 0) * 64 + lane) * 4 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        pi[k] = __hip_atomic_load(p + ((b * 2 + 1) * 64 + lane) * 4 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
-      interp_epilogue<OM, CONJ, ACC>(a, rows, b, re, im, lane);
+      if (q == 0) {
+        re = pr;
+        im = pi;
+      } else {
+        re += pr;
+        im += pi;
+      }
     }
+    interp_epilogue<OM, CONJ, ACC>(a, rows, b, re, im, lane);
   }
 }
 
@@ -509,45 +529,39 @@ __device__ __forceinline__ void regular_unit(const InterpArgs& a, const WorkItem
   }
   const TileRsrc<NB> r = make_tile_rsrc<NB, IL>(a);
   stage_cols(r, u.tile_begin, u.tile_end - u.tile_begin, stage, lane);
-  stream_tiles<KIND, NB, IL, true, OM, CONJ, ACC>(a, r, u.tile_begin, u.tile_end, 1, u.group, u.group_end,
-                                                  acc_re, acc_im, lane, stage);
+  stream_tiles<KIND, NB, IL, true, OM, CONJ, ACC>(a, r, u.tile_begin, u.tile_end, u.group, u.group_end, acc_re,
+                                                  acc_im, lane, stage);
 }
 
-// Blocks of kTileWaves waves.  The first n_long blocks each own one long group
-// (waves stride over its tiles and are summed through LDS in wave order); the
-// other blocks hold one work unit (a run of whole groups) per wave, the units
-// of one XCD contiguous in pack order.
+// One wave per block, one entry of the launch order each (a regular unit, a
+// run of whole groups, or a chunk of a long group), the entries of one XCD
+// contiguous in pack order.  One-wave blocks let the dispatcher start a new
+// entry whenever a wave slot frees, so uneven entries balance themselves
+// across the CUs.
 template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC>
-__global__ __launch_bounds__(kTileWaves * 64) void interp_tiles_kernel(InterpArgs a) {
-  // long-group blocks reduce through it; regular blocks stage column words in it
-  __shared__ float4_t red[kTileWaves][NB][2][64];
-  static_assert(sizeof(red) >= sizeof(uint32_t) * kStageWords * kTileWaves, "LDS stage does not fit");
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+__global__ __launch_bounds__(64) void interp_tiles_kernel(InterpArgs a) {
+  __shared__ uint32_t stage[kStageWords];
+  const int lane = threadIdx.x;
   float4_t acc_re[NB], acc_im[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     acc_re[b] = float4_t{0.f, 0.f, 0.f, 0.f};
     acc_im[b] = float4_t{0.f, 0.f, 0.f, 0.f};
   }
-  if (blockIdx.x < a.n_long) {
-    const WorkItem it = a.long_items[blockIdx.x];
-    if (it.group < 0) return;  // padding block: uniform exit before the barrier
+  const uint32_t entry = a.order[xcd_remap(blockIdx.x, gridDim.x)];
+  if (entry & kChunkTag) {
+    const ChunkItem c = a.chunks[entry & ~kChunkTag];
     if constexpr (NB == 1 && IL) {
       if (a.problems) {
-        const InterpArgs v = problem_args(a, a.long_problem[blockIdx.x]);
-        long_group<KIND, NB, IL, OM, CONJ, ACC>(v, it, red, acc_re, acc_im, lane, wave);
+        long_chunk<KIND, NB, IL, OM, CONJ, ACC>(problem_args(a, c.problem), c, acc_re, acc_im, lane, stage);
         return;
       }
     }
-    long_group<KIND, NB, IL, OM, CONJ, ACC>(a, it, red, acc_re, acc_im, lane, wave);
+    long_chunk<KIND, NB, IL, OM, CONJ, ACC>(a, c, acc_re, acc_im, lane, stage);
     return;
   }
-  const uint32_t rb = blockIdx.x - a.n_long;
-  const uint32_t item_id = xcd_remap(rb, gridDim.x - a.n_long) * kTileWaves + wave;
-  if (item_id >= a.n_items) return;
+  const uint32_t item_id = entry;
   const WorkItem u = a.items[item_id];
-  uint32_t* stage = reinterpret_cast<uint32_t*>(&red[0][0][0][0]) + wave * kStageWords;
   if constexpr (NB == 1 && IL) {
     if (a.problems) {
       regular_unit<KIND, NB, IL, OM, CONJ, ACC>(problem_args(a, a.item_problem[item_id]), u, acc_re, acc_im,
@@ -742,18 +756,21 @@ __global__ __launch_bounds__(256) void dc_residual_kernel(const float* __restric
 
 // ------------------------------------------------------------- dispatch
 
+template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC>
+static void launch_both(const InterpArgs& a, hipStream_t s) {
+  const uint32_t blocks = a.n_chunks + a.n_items;
+  if (blocks) hipLaunchKernelGGL((interp_tiles_kernel<KIND, NB, IL, OM, CONJ, ACC>), dim3(blocks), dim3(64), 0, s, a);
+}
+
 template <int KIND, int NB, bool IL, int OM>
 static void launch_tiles_om(const InterpArgs& a, bool conj, bool acc, hipStream_t s) {
-  const uint32_t blocks = a.n_long + div_up(a.n_items, kTileWaves);
-  if (!blocks) return;
-  const dim3 grid(blocks), block(kTileWaves * 64);
   constexpr bool kAccOk = OM != kOutSplit16 && OM != kOutSplit16IL && OM != kOutSplitBF16IL;
   if (conj) {
-    if (kAccOk && acc) hipLaunchKernelGGL((interp_tiles_kernel<KIND, NB, IL, OM, true, kAccOk>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((interp_tiles_kernel<KIND, NB, IL, OM, true, false>), grid, block, 0, s, a);
+    if (kAccOk && acc) launch_both<KIND, NB, IL, OM, true, kAccOk>(a, s);
+    else launch_both<KIND, NB, IL, OM, true, false>(a, s);
   } else {
-    if (kAccOk && acc) hipLaunchKernelGGL((interp_tiles_kernel<KIND, NB, IL, OM, false, kAccOk>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((interp_tiles_kernel<KIND, NB, IL, OM, false, false>), grid, block, 0, s, a);
+    if (kAccOk && acc) launch_both<KIND, NB, IL, OM, false, kAccOk>(a, s);
+    else launch_both<KIND, NB, IL, OM, false, false>(a, s);
   }
 }
 
@@ -814,8 +831,11 @@ static int launch_interp_impl(const tsn_direction* d, int kind, const void* hi, 
   InterpArgs a{};
   a.items = d->items;
   a.n_items = d->n_items;
-  a.long_items = d->long_items;
-  a.n_long = d->n_long;
+  a.chunks = d->chunks;
+  a.n_chunks = d->n_chunks;
+  a.order = d->order;
+  a.partials = d->partials;
+  a.counters = d->counters;
   a.group_rows = d->group_rows;
   a.group_offsets = d->group_offsets;
   a.tile_cols = d->tile_cols;
@@ -999,7 +1019,7 @@ static int device_copy(const T* host, size_t count, T** out, size_t* bytes) {
 static void release(tsn_direction* d) {
   if (!d) return;
   void* ptrs[] = {d->group_offsets, d->group_rows, d->tile_cols, d->tile_a, d->tile_idx,
-                  d->tile_dense, d->items, d->long_items, d->row_ptr, d->row_ids,
+                  d->tile_dense, d->items, d->order, d->chunks, d->partials, d->counters, d->row_ptr, d->row_ids,
                   d->cols, d->vals, d->short_rows, d->long_rows};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1007,16 +1027,15 @@ static void release(tsn_direction* d) {
   delete d;
 }
 
-// Work schedule of a packed direction.  Groups longer than kSplitAboveTiles
-// tiles are units owned by a whole block (padded with group -1 entries to a
-// multiple of 8 blocks so the other blocks keep their XCD mapping).  The other
-// groups are cut, in pack order, into contiguous runs of whole groups holding
-// at most unit_tiles tiles (a single group is never cut); an empty group is a
-// unit of its own (zero epilogue).  unit_tiles 0 picks the default: the
-// regular tiles spread over 1.5 rounds of resident waves (one 16-wave block
-// per CU), clamped to 2..8 tiles.  Measured at 205,824 samples / 512^2 grid:
-// 8-tile units beat both one full round (12-tile units leave a tiny second
-// round) and short units (more per-unit start-up), 26 us against 35 us.
+// Work schedule of a packed direction: the groups of at most
+// kSplitAboveTiles tiles are cut, in pack order, into contiguous runs of whole
+// groups holding at most unit_tiles tiles (a single group is never cut); an
+// empty group is a unit of its own (zero epilogue).  Longer groups are
+// chunks, fixed at upload.  unit_tiles 0 picks the default: the regular tiles
+// over 12 waves per CU, clamped to 4..14 tiles.  Measured at 205,824 samples /
+// 512^2 grid (one-wave blocks, pack-ordered launch): G 18.7 us with 14-tile
+// units against 19.5 us with 8 or 12; G^T 20.7 us against 23.7 (8) and
+// 21.5 (12); 4-tile units pay more per-unit start-up.
 static int schedule_units(tsn_direction* d, uint32_t unit_tiles) {
   const std::vector<uint32_t>& off = d->host_offsets;
   const uint32_t groups = d->h.groups;
@@ -1030,28 +1049,34 @@ static int schedule_units(tsn_direction* d, uint32_t unit_tiles) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device) != hipSuccess || cus <= 0)
       cus = 256;
-    const uint64_t waves = static_cast<uint64_t>(cus) * kTileWaves * 3 / 2;
-    target = std::clamp<int64_t>(static_cast<int64_t>((regular_tiles + waves - 1) / waves), 2, 8);
+    const uint64_t waves = static_cast<uint64_t>(cus) * 12;
+    target = std::clamp<int64_t>(static_cast<int64_t>((regular_tiles + waves - 1) / waves), 4, 14);
   }
   // a unit's column words are staged in the wave's LDS slice: at most 16 tiles
   target = std::min<int64_t>(target, kSplitAboveTiles);
-  std::vector<WorkItem> items, long_items;
+  std::vector<WorkItem> items;
+  std::vector<uint32_t> order;
+  uint32_t chunk = 0;  // chunks are numbered in group order (long_chunks)
   WorkItem cur{-1, -1, 0, 0};
   auto flush = [&]() {
-    if (cur.group >= 0) items.push_back(cur);
+    if (cur.group >= 0) {
+      order.push_back(static_cast<uint32_t>(items.size()));
+      items.push_back(cur);
+    }
     cur = {-1, -1, 0, 0};
   };
   for (uint32_t g = 0; g < groups; ++g) {
     const int32_t b = static_cast<int32_t>(off[g]);
     const int32_t e = static_cast<int32_t>(off[g + 1]);
     const int32_t gi = static_cast<int32_t>(g);
-    if (e - b > kSplitAboveTiles) {
+    if (e - b > kSplitAboveTiles) {  // a chunked long group
       flush();
-      long_items.push_back({gi, gi + 1, b, e});
+      for (int32_t t = b; t < e; t += kSplitAboveTiles) order.push_back(kChunkTag | chunk++);
       continue;
     }
     if (e == b) {
       flush();
+      order.push_back(static_cast<uint32_t>(items.size()));
       items.push_back({gi, gi + 1, b, e});
       continue;
     }
@@ -1064,11 +1089,11 @@ static int schedule_units(tsn_direction* d, uint32_t unit_tiles) {
     }
   }
   flush();
-  while (long_items.size() % 8) long_items.push_back({-1, -1, 0, 0});
-  WorkItem *new_items = nullptr, *new_long = nullptr;
+  WorkItem* new_items = nullptr;
+  uint32_t* new_order = nullptr;
   size_t bytes = 0;
   int rc = device_copy(items.data(), items.size(), &new_items, &bytes);
-  if (rc == TSN_OK) rc = device_copy(long_items.data(), long_items.size(), &new_long, &bytes);
+  if (rc == TSN_OK) rc = device_copy(order.data(), order.size(), &new_order, &bytes);
   if (rc != TSN_OK) {
     if (new_items) (void)hipFree(new_items);
     return rc;
@@ -1076,15 +1101,56 @@ static int schedule_units(tsn_direction* d, uint32_t unit_tiles) {
   // Earlier schedules stay allocated until the direction is freed: a captured
   // graph may still hold their addresses in its kernel arguments.
   if (d->items) d->retired.push_back(d->items);
-  if (d->long_items) d->retired.push_back(d->long_items);
+  if (d->order) d->retired.push_back(d->order);
   d->items = new_items;
-  d->long_items = new_long;
+  d->order = new_order;
   d->n_items = static_cast<uint32_t>(items.size());
-  d->n_long = static_cast<uint32_t>(long_items.size());
   d->unit_tiles = static_cast<uint32_t>(target);
   d->host_items = items;
-  d->host_long = long_items;
+  d->host_order = order;
   d->bytes += bytes;
+  return TSN_OK;
+}
+
+}  // namespace tsn
+
+std::vector<tsn::ChunkItem> tsn::long_chunks(const std::vector<uint32_t>& off, uint32_t problem, uint32_t slot_base,
+                                             uint32_t counter_base, uint32_t* slots, uint32_t* counters) {
+  std::vector<ChunkItem> out;
+  uint32_t slot = slot_base, counter = counter_base;
+  for (size_t g = 0; g + 1 < off.size(); ++g) {
+    const uint32_t b = off[g], e = off[g + 1];
+    if (e - b <= static_cast<uint32_t>(kSplitAboveTiles)) continue;
+    const uint32_t n = (e - b + kSplitAboveTiles - 1) / kSplitAboveTiles;
+    for (uint32_t c = 0; c < n; ++c) {
+      const uint32_t cb = b + c * kSplitAboveTiles;
+      out.push_back({static_cast<int32_t>(g), static_cast<int32_t>(cb),
+                     static_cast<int32_t>(std::min<uint32_t>(e, cb + kSplitAboveTiles)), c, n, slot, counter,
+                     problem});
+    }
+    slot += n;
+    ++counter;
+  }
+  *slots = slot - slot_base;
+  *counters = counter - counter_base;
+  return out;
+}
+
+namespace tsn {
+
+// Chunk table, partial slots and zeroed arrival counters of a direction's
+// long groups (fixed by the pack; rescheduling keeps them).
+static int upload_chunks(tsn_direction* d) {
+  uint32_t slots = 0, counters = 0;
+  const std::vector<ChunkItem> chunks = long_chunks(d->host_offsets, 0, 0, 0, &slots, &counters);
+  if (chunks.empty()) return TSN_OK;
+  size_t bytes = 0;
+  TSN_CHECK(device_copy(chunks.data(), chunks.size(), &d->chunks, &bytes));
+  d->n_chunks = static_cast<uint32_t>(chunks.size());
+  TSN_HIP(hipMalloc(reinterpret_cast<void**>(&d->partials), static_cast<size_t>(slots) * kSlotFloats * 4));
+  TSN_HIP(hipMalloc(reinterpret_cast<void**>(&d->counters), counters * sizeof(uint32_t)));
+  TSN_HIP(hipMemset(d->counters, 0, counters * sizeof(uint32_t)));
+  d->bytes += bytes + static_cast<size_t>(slots) * kSlotFloats * 4 + counters * sizeof(uint32_t);
   return TSN_OK;
 }
 
@@ -1158,6 +1224,7 @@ static int upload_pack(const HostPack& p, tsn_direction** out) {
   if (rc == TSN_OK) rc = device_copy(tile_dense.data(), tile_dense.size(), &d->tile_dense, &bytes);
   d->bytes = bytes;
   if (rc == TSN_OK) rc = schedule_units(d, 0);
+  if (rc == TSN_OK) rc = upload_chunks(d);
   if (rc != TSN_OK) {
     release(d);
     return rc;
@@ -1196,43 +1263,55 @@ int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* o
   std::vector<const tsn_direction*> dirs;
   for (const InterpBatchPart& p : parts)
     if (std::find(dirs.begin(), dirs.end(), p.dir) == dirs.end()) dirs.push_back(p.dir);
-  std::vector<WorkItem> items, long_items;
-  std::vector<uint32_t> item_problem, long_problem;
+  std::vector<WorkItem> items;
+  std::vector<uint32_t> item_problem, order;
+  std::vector<ChunkItem> chunks;
+  uint32_t slots = 0, counters = 0;
   for (const tsn_direction* d : dirs) {
     std::vector<uint32_t> members;
     for (uint32_t i = 0; i < parts.size(); ++i)
       if (parts[i].dir == d) members.push_back(i);
-    for (const WorkItem& u : d->host_items)
-      for (uint32_t m : members) {
-        items.push_back(u);
-        item_problem.push_back(m);
+    // every part has its own slots and counters
+    uint32_t ns = 0, nc = 0;
+    const std::vector<ChunkItem> dc = long_chunks(d->host_offsets, 0, 0, 0, &ns, &nc);
+    for (uint32_t e : d->host_order)
+      for (uint32_t k = 0; k < members.size(); ++k) {
+        if (e & kChunkTag) {
+          ChunkItem c = dc[e & ~kChunkTag];
+          c.slot0 += slots + k * ns;
+          c.counter += counters + k * nc;
+          c.problem = members[k];
+          order.push_back(kChunkTag | static_cast<uint32_t>(chunks.size()));
+          chunks.push_back(c);
+        } else {
+          order.push_back(static_cast<uint32_t>(items.size()));
+          items.push_back(d->host_items[e]);
+          item_problem.push_back(members[k]);
+        }
       }
-    for (const WorkItem& l : d->host_long) {
-      if (l.group < 0) continue;  // the direction's own padding
-      for (uint32_t m : members) {
-        long_items.push_back(l);
-        long_problem.push_back(m);
-      }
-    }
-  }
-  while (long_items.size() % 8) {
-    long_items.push_back({-1, -1, 0, 0});
-    long_problem.push_back(0);
+    slots += static_cast<uint32_t>(members.size()) * ns;
+    counters += static_cast<uint32_t>(members.size()) * nc;
   }
   InterpBatch b;
   b.values = values;
   b.dense = std::all_of(parts.begin(), parts.end(), [](const InterpBatchPart& p) { return p.dir->tile_dense; });
   b.n_problems = static_cast<uint32_t>(probs.size());
   b.n_items = static_cast<uint32_t>(items.size());
-  b.n_long = static_cast<uint32_t>(long_items.size());
+  b.n_chunks = static_cast<uint32_t>(chunks.size());
   size_t bytes = 0;
   InterpProblem* dprobs = nullptr;
   int rc = device_copy(probs.data(), probs.size(), &dprobs, &bytes);
   b.problems = dprobs;
   if (rc == TSN_OK) rc = device_copy(items.data(), items.size(), &b.items, &bytes);
   if (rc == TSN_OK) rc = device_copy(item_problem.data(), item_problem.size(), &b.item_problem, &bytes);
-  if (rc == TSN_OK) rc = device_copy(long_items.data(), long_items.size(), &b.long_items, &bytes);
-  if (rc == TSN_OK) rc = device_copy(long_problem.data(), long_problem.size(), &b.long_problem, &bytes);
+  if (rc == TSN_OK) rc = device_copy(chunks.data(), chunks.size(), &b.chunks, &bytes);
+  if (rc == TSN_OK) rc = device_copy(order.data(), order.size(), &b.order, &bytes);
+  if (rc == TSN_OK && slots) {
+    const hipError_t e1 = hipMalloc(reinterpret_cast<void**>(&b.partials), static_cast<size_t>(slots) * kSlotFloats * 4);
+    const hipError_t e2 = e1 == hipSuccess ? hipMalloc(reinterpret_cast<void**>(&b.counters), counters * 4) : e1;
+    const hipError_t e3 = e2 == hipSuccess ? hipMemset(b.counters, 0, counters * 4) : e2;
+    if (e3 != hipSuccess) rc = fail(TSN_ERR_RUNTIME, "batch chunk buffers: %s", hipGetErrorString(e3));
+  }
   if (rc != TSN_OK) {
     interp_batch_free(&b);
     return rc;
@@ -1243,7 +1322,7 @@ int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* o
 
 void interp_batch_free(InterpBatch* b) {
   if (!b) return;
-  void* ptrs[] = {b->problems, b->items, b->item_problem, b->long_items, b->long_problem};
+  void* ptrs[] = {b->problems, b->items, b->item_problem, b->chunks, b->partials, b->counters, b->order};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   *b = InterpBatch{};
@@ -1266,12 +1345,14 @@ int launch_interp_batch(const InterpBatch& b, int kind, int out_mode, bool conj,
   InterpArgs a{};
   a.items = b.items;
   a.n_items = b.n_items;
-  a.long_items = b.long_items;
-  a.n_long = b.n_long;
+  a.chunks = b.chunks;
+  a.n_chunks = b.n_chunks;
+  a.order = b.order;
+  a.partials = b.partials;
+  a.counters = b.counters;
   a.scale = scale;
   a.problems = static_cast<const InterpProblem*>(b.problems);
   a.item_problem = b.item_problem;
-  a.long_problem = b.long_problem;
   a.coils = kCoilsPerBlock;  // replaced per problem
   switch (kind) {
     case kSparseSplit: launch_batch_kind<kSparseSplit>(a, out_mode, conj, s); break;
@@ -1496,12 +1577,17 @@ int tsn_interp_f32(const tsn_direction* d, const void* in, void* out, uint32_t c
 
 int tsn_interp_il(const tsn_direction* d, const void* panel, void* out, uint32_t coils, int conjugate,
                   int accumulate, float scale, int out_layout, void* stream) {
-  if (out_layout != TSN_OUT_COIL_MAJOR && out_layout != TSN_OUT_ROW_MAJOR)
+  if (out_layout != TSN_OUT_COIL_MAJOR && out_layout != TSN_OUT_ROW_MAJOR && out_layout != TSN_OUT_SPLIT_IL)
     return fail(TSN_ERR_ARGUMENT, "unknown output layout %d", out_layout);
   if (!d) return fail(TSN_ERR_ARGUMENT, "null direction");
-  return launch_interp(d, d->values == TSN_VALUES_BF16 ? kSparseSplitBF16 : kSparseSplit, panel, nullptr, coils,
-                       out_layout, conjugate != 0, accumulate != 0, scale, static_cast<float*>(out), nullptr,
-                       nullptr, static_cast<hipStream_t>(stream), true, nullptr);
+  const int kind = d->values == TSN_VALUES_BF16 ? kSparseSplitBF16 : kSparseSplit;
+  if (out_layout == TSN_OUT_SPLIT_IL) {
+    if (accumulate) return fail(TSN_ERR_ARGUMENT, "a split-panel output cannot accumulate");
+    return launch_interp(d, kind, panel, nullptr, coils, kOutSplit16IL, conjugate != 0, false, scale, nullptr,
+                         nullptr, nullptr, static_cast<hipStream_t>(stream), true, static_cast<uint32_t*>(out));
+  }
+  return launch_interp(d, kind, panel, nullptr, coils, out_layout, conjugate != 0, accumulate != 0, scale,
+                       static_cast<float*>(out), nullptr, nullptr, static_cast<hipStream_t>(stream), true, nullptr);
 }
 
 int tsn_interp_dc(const tsn_direction* d, const void* panel, const void* measurements, int measurement_layout,

@@ -20,8 +20,7 @@ constexpr int kKComp = 16;  // kept values per tile row
 constexpr int kN = 16;      // channels per panel block (8 coils x Re/Im)
 constexpr int kCoilsPerBlock = 8;
 constexpr int kMaxCoils = 16;
-constexpr int kTileWaves = 16;        // waves per interpolation block (1024 threads)
-constexpr int kSplitAboveTiles = 16;  // longer groups get a whole block (waves stride over tiles)
+constexpr int kSplitAboveTiles = 16;  // tiles per unit at most; longer groups are cut into chunks of this size
 constexpr int kLongRowEntries = 128;  // FP32 CSR rows longer than this get a whole block
 // Device column words of packed tiles hold column * 32 (a byte offset) below
 // bit 31, so a pack may have at most 2^26 columns.
@@ -86,16 +85,39 @@ inline void decode_row_meta(const uint32_t* pair_meta, uint32_t* row_meta) {
 
 // Work unit of the interpolation kernels: whole groups [group, group_end)
 // whose tiles [tile_begin, tile_end) one wave streams through a continuous
-// pipeline (epilogue at every group boundary).  Units are sized to a target
-// tile count so one round of waves covers the launch.  Groups longer than
-// kSplitAboveTiles are separate units owned by a whole block; group < 0 marks
-// a padding unit.
+// pipeline (epilogue at every group boundary).  Units hold at most a target
+// tile count; groups longer than kSplitAboveTiles are cut into chunks.
 struct WorkItem {
   int32_t group;
   int32_t group_end;
   int32_t tile_begin;
   int32_t tile_end;
 };
+
+// One chunk of a long group: kSplitAboveTiles consecutive tiles of it (the
+// last chunk fewer).  Each chunk's wave publishes its partial sums to slot
+// slot0 + chunk; the wave that brings the group's arrival counter to
+// n_chunks adds the slots in chunk order and writes the rows, so the sums do
+// not depend on the schedule or on which wave arrives last.
+struct ChunkItem {
+  int32_t group;
+  int32_t tile_begin, tile_end;
+  uint32_t chunk, n_chunks;
+  uint32_t slot0;    // partial slot of the group's chunk 0
+  uint32_t counter;  // arrival counter of the group (reset to 0 by the last arrival)
+  uint32_t problem;  // batched launches: the problem table entry
+};
+// Floats of one partial slot: two coil blocks x (re, im) x 64 lanes x float4.
+constexpr uint32_t kSlotFloats = 2 * 2 * 64 * 4;
+// Launch order entries: a unit index, or kChunkTag | chunk index.  Units and
+// chunks follow pack order, so each XCD's contiguous range of the launch
+// holds neighbouring groups (shared columns in its L2) and its share of the
+// long groups.
+constexpr uint32_t kChunkTag = 0x80000000u;
+// Chunks of a direction's long groups with slots and counters numbered from
+// slot_base / counter_base (host side).
+std::vector<ChunkItem> long_chunks(const std::vector<uint32_t>& group_offsets, uint32_t problem, uint32_t slot_base,
+                                   uint32_t counter_base, uint32_t* slots, uint32_t* counters);
 // One coil block of one problem of a batched interpolation launch.
 struct InterpBatchPart {
   const tsn_direction* dir;
@@ -113,9 +135,11 @@ struct InterpBatch {
   void* problems = nullptr;
   WorkItem* items = nullptr;
   uint32_t* item_problem = nullptr;
-  WorkItem* long_items = nullptr;
-  uint32_t* long_problem = nullptr;
-  uint32_t n_problems = 0, n_items = 0, n_long = 0;
+  ChunkItem* chunks = nullptr;   // long-group chunks of every problem
+  float* partials = nullptr;     // their partial slots
+  uint32_t* counters = nullptr;  // one per problem x long group, zero between launches
+  uint32_t* order = nullptr;     // launch order of units and chunks (kChunkTag marks a chunk)
+  uint32_t n_problems = 0, n_items = 0, n_chunks = 0;
   int values = TSN_VALUES_FP16;
   bool dense = false;
 };
@@ -168,11 +192,15 @@ struct tsn_direction {
   uint16_t* tile_idx = nullptr;      // [tiles * 64] per lane sparse index byte re | im << 8
   uint4* tile_dense = nullptr;       // [tiles * 64 * 2] per lane {re 8xf16}, {im 8xf16}
   tsn::WorkItem* items = nullptr;    // runs of groups with <= kSplitAboveTiles tiles each
-  tsn::WorkItem* long_items = nullptr;  // longer groups, padded to a multiple of 8
-  uint32_t n_items = 0, n_long = 0;
+  tsn::ChunkItem* chunks = nullptr;  // chunks of the longer groups (fixed by the pack)
+  float* partials = nullptr;         // [slots][kSlotFloats] partial sums of the chunks
+  uint32_t* counters = nullptr;      // [long groups] arrival counters, zero between launches
+  uint32_t n_items = 0, n_chunks = 0;
   uint32_t unit_tiles = 0;           // tile budget of one regular unit
   std::vector<uint32_t> host_offsets;  // group_offsets kept for rescheduling
-  std::vector<tsn::WorkItem> host_items, host_long;  // the current schedule (batched launches merge it)
+  uint32_t* order = nullptr;         // [n_items + n_chunks] launch order (kChunkTag marks chunks)
+  std::vector<tsn::WorkItem> host_items;  // the current schedule (batched launches merge it)
+  std::vector<uint32_t> host_order;
   std::vector<void*> retired;          // earlier schedules (graphs may reference them)
   // ---- FP32 CSR (device)
   uint32_t csr_rows = 0;

@@ -23,10 +23,10 @@ import os
 import torch
 
 from . import _lib
-from ._lib import OUT_COIL_MAJOR, OUT_ROW_MAJOR, PackHeader, check, lib
+from ._lib import OUT_COIL_MAJOR, OUT_ROW_MAJOR, OUT_SPLIT_IL, PackHeader, check, lib
 from .pack import HEADER_FIELDS, HostPack, KbMatrix
 
-LAYOUTS = {"coil_major": OUT_COIL_MAJOR, "row_major": OUT_ROW_MAJOR}
+LAYOUTS = {"coil_major": OUT_COIL_MAJOR, "row_major": OUT_ROW_MAJOR, "split_il": OUT_SPLIT_IL}
 
 
 def _stream() -> C.c_void_p:
@@ -176,10 +176,15 @@ def split_panel_il(samples: torch.Tensor, coils: int, scale: float = 1.0, values
 def interp_il(direction: Direction, panel: torch.Tensor, out: torch.Tensor, coils: int = 8,
               conjugate: bool = False, accumulate: bool = False, scale: float = 1.0,
               layout: str = "coil_major") -> torch.Tensor:
-    """Packed interpolation of an interleaved split panel (int32 [blocks, n_cols, 16])."""
+    """Packed interpolation of an interleaved split panel (int32 [blocks, n_cols, 16]).
+    ``layout="split_il"`` writes ``out`` as such a panel of the output rows
+    (int32 [blocks, rows, 16]); the other layouts write complex64."""
 
     _require(panel, torch.int32, coil_blocks(coils) * direction.cols * 16, "panel")
-    _require(out, torch.complex64, coils * direction.rows, "out")
+    if layout == "split_il":
+        _require(out, torch.int32, coil_blocks(coils) * direction.rows * 16, "out")
+    else:
+        _require(out, torch.complex64, coils * direction.rows, "out")
     check(lib().tsn_interp_il(direction.handle, _ptr(panel), _ptr(out), coils, int(conjugate), int(accumulate),
                               float(scale), LAYOUTS[layout], _stream()))
     return out
