@@ -32,7 +32,8 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
   __shared__ double red[512];
   constexpr int K = 64 * R;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint32_t n1 = blockIdx.x, chunk = blockIdx.y;
+  // neighbouring columns on one XCD: their 64-byte rows of T fill whole L2 lines
+  const uint32_t n1 = xcd_remap(blockIdx.x, gridDim.x), chunk = blockIdx.y;
   const uint32_t c = chunk * 8 + w;
   const bool active = c < a.coils;
   const WaveTwiddles<R> twp = load_twiddles<R>(a.tw0, lane);
@@ -89,20 +90,8 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
   __syncthreads();
   emit_exchange<R, false>(out, lds, lane, w);
   __syncthreads();
-  const uint32_t cw = min(8u, a.coils - chunk * 8);
-  for (uint32_t k = threadIdx.x; k < static_cast<uint32_t>(K); k += blockDim.x) {
-    float2* dst = a.t_fwd + (static_cast<size_t>(k) * a.n1 + n1) * a.cstride + chunk * 8;
-    if (cw == 8) {
-      float4* d4 = reinterpret_cast<float4*>(dst);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float2 u = lds[xslot(k, 2 * i)], v = lds[xslot(k, 2 * i + 1)];
-        d4[i] = make_float4(u.x, u.y, v.x, v.y);
-      }
-    } else {
-      for (uint32_t i = 0; i < cw; ++i) dst[i] = lds[xslot(k, i)];
-    }
-  }
+  store_rows<K>(a.t_fwd + static_cast<size_t>(n1) * a.cstride + chunk * 8,
+                static_cast<uint64_t>(a.n1) * a.cstride, min(8u, a.coils - chunk * 8), a.cstride, lds);
 }
 
 // Forward, second axis: grid row k0 -> panel rows k0*K1 + k1.  OUT 0: FP16
@@ -135,33 +124,31 @@ __global__ __launch_bounds__(512) void fwd_axis1_kernel(FftArgs args) {
   emit_exchange<R, false>(out, lds, lane, w);
   __syncthreads();
   const uint64_t grid_rows = static_cast<uint64_t>(a.k0) * K;
-  for (uint32_t k1 = threadIdx.x; k1 < static_cast<uint32_t>(K); k1 += blockDim.x) {
-    const uint64_t r = static_cast<uint64_t>(k0) * K + k1;
-    if constexpr (OUT == 0 || OUT == 2) {
-      uint32_t w[16];  // channel c = Re coil c, c + 8 = Im coil c; hi | res << 16
+  if constexpr (OUT == 0 || OUT == 2) {
+    // four threads per panel row, one uint4 each (words 4q .. 4q + 3: Re for
+    // q < 2, else Im, of coils 4 (q & 1) .. + 3; hi | res << 16), so a wave's
+    // store covers 16 whole 64-byte rows
+    uint4* base = reinterpret_cast<uint4*>(a.panel_il + (chunk * grid_rows + static_cast<uint64_t>(k0) * K) * 16);
+    for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(K) * 4; e += blockDim.x) {
+      const uint32_t k1 = e >> 2, q = e & 3;
+      uint32_t w4[4];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const float2 v = lds[xslot(k1, c)];
+      for (int i = 0; i < 4; ++i) {
+        const float2 v = lds[xslot(k1, 4 * (q & 1) + i)];
+        const float x = q < 2 ? v.x : v.y;
         if constexpr (OUT == 2) {
-          w[c] = split_bf16(v.x);
-          w[c + 8] = split_bf16(v.y);
+          w4[i] = split_bf16(x);
         } else {
           _Float16 h, r2;
-          split16(v.x, h, r2);
-          w[c] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
-                 (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r2)) << 16);
-          split16(v.y, h, r2);
-          w[c + 8] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
-                     (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r2)) << 16);
+          split16(x, h, r2);
+          w4[i] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
+                  (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r2)) << 16);
         }
       }
-      uint4* d4 = reinterpret_cast<uint4*>(a.panel_il + (chunk * grid_rows + r) * 16);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) d4[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
-    } else {
-      float2* dst = a.panel32 + r * a.cstride + chunk * 8;
-      for (uint32_t i = 0; i < cw; ++i) dst[i] = lds[xslot(k1, i)];
+      base[static_cast<size_t>(k1) * 4 + q] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
     }
+  } else {
+    store_rows<K>(a.panel32 + static_cast<uint64_t>(k0) * K * a.cstride + chunk * 8, a.cstride, cw, a.cstride, lds);
   }
 }
 
@@ -175,7 +162,8 @@ __global__ __launch_bounds__(512) void inv_axis1_kernel(FftArgs args) {
   extern __shared__ float2 lds[];
   constexpr int K = 64 * R;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint32_t k0 = blockIdx.x, chunk = blockIdx.y;
+  // neighbouring grid rows on one XCD: their 64-byte rows of T2 fill whole L2 lines
+  const uint32_t k0 = xcd_remap(blockIdx.x, gridDim.x), chunk = blockIdx.y;
   const uint32_t cw = min(8u, a.coils - chunk * 8);
   fill_exchange<K>(a.grid + static_cast<size_t>(k0) * K * a.cstride + chunk * 8, a.cstride, cw, lds);
   __syncthreads();
@@ -189,19 +177,8 @@ __global__ __launch_bounds__(512) void inv_axis1_kernel(FftArgs args) {
   __syncthreads();
   emit_exchange<R, true>(out, lds, lane, w);
   __syncthreads();
-  for (uint32_t n1 = threadIdx.x; n1 < a.n1; n1 += blockDim.x) {
-    float2* dst = a.t_inv + (static_cast<size_t>(n1) * a.k0 + k0) * a.cstride + chunk * 8;
-    if (cw == 8) {
-      float4* d4 = reinterpret_cast<float4*>(dst);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float2 u = lds[xslot(n1, 2 * i)], v = lds[xslot(n1, 2 * i + 1)];
-        d4[i] = make_float4(u.x, u.y, v.x, v.y);
-      }
-    } else {
-      for (uint32_t i = 0; i < cw; ++i) dst[i] = lds[xslot(n1, i)];
-    }
-  }
+  store_rows<K / 2>(a.t_inv + static_cast<size_t>(k0) * a.cstride + chunk * 8,
+                    static_cast<uint64_t>(a.k0) * a.cstride, cw, a.cstride, lds);
 }
 
 // Inverse, first grid axis for image column n1: IDFT along k0, keep n0 < N0,

@@ -311,6 +311,27 @@ __device__ __forceinline__ void fill_exchange(const float2* __restrict__ src, ui
   }
 }
 
+// Store ROWS rows of the exchange area (8 coils each, coils >= cw skipped) to
+// dst + r * row_stride (float2 units).  Whole 8-coil rows at a 16-byte
+// aligned stride go out as four float4 per row from four neighbouring
+// threads, so a wave's store covers 16 whole 64-byte rows; otherwise one
+// thread writes a row coil by coil.  512-thread blocks.
+template <int ROWS>
+__device__ __forceinline__ void store_rows(float2* __restrict__ dst, uint64_t row_stride, uint32_t cw,
+                                           uint32_t cstride, const float2* lds) {
+  if (cw == 8 && (cstride & 1u) == 0) {
+    for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(ROWS) * 4; e += blockDim.x) {
+      const uint32_t r = e >> 2, q = e & 3;
+      const float2 u = lds[xslot(static_cast<int>(r), static_cast<int>(2 * q))];
+      const float2 v = lds[xslot(static_cast<int>(r), static_cast<int>(2 * q + 1))];
+      reinterpret_cast<float4*>(dst + r * row_stride)[q] = make_float4(u.x, u.y, v.x, v.y);
+    }
+  } else {
+    for (uint32_t r = threadIdx.x; r < static_cast<uint32_t>(ROWS); r += blockDim.x)
+      for (uint32_t i = 0; i < cw; ++i) dst[r * row_stride + i] = lds[xslot(static_cast<int>(r), static_cast<int>(i))];
+  }
+}
+
 // Write the wave's outputs to the block exchange area, element w of row k
 // (k < K, or k < K/2 when HALF_OUT).  Call between two __syncthreads.
 template <int R, bool HALF_OUT>

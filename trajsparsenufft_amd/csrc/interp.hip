@@ -34,12 +34,6 @@ enum OutMode { kOutCoilMajor = 0, kOutRowMajor = 1, kOutSplit16 = 2, kOutSplit16
 // MFMA, FP16 hi only, and BF16 tiles x BF16 hi + residual on smfmac.
 enum TileKind { kSparseSplit = 0, kDenseSplit = 1, kSparseHi = 2, kSparseSplitBF16 = 3 };
 
-// Block b runs on XCD b % 8; give each XCD a contiguous range of items.
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
-  const uint32_t xcd = b & 7u, idx = b >> 3;
-  const uint32_t per = nb >> 3, rem = nb & 7u;
-  return xcd * per + min(xcd, rem) + idx;
-}
 
 struct InterpArgs {
   const WorkItem* items;

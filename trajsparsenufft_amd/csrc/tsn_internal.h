@@ -47,6 +47,15 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a
 
 // bfloat16 bits of v, round to nearest even: the top half of the float32 bits
 // after adding 0x7FFF plus the lowest kept bit; NaN stays a quiet NaN.
+// Block b of a launch runs on XCD b % 8 (the dispatcher's round robin); this
+// gives each XCD a contiguous range of the nb work indices (a permutation of
+// 0 .. nb-1), so neighbouring work shares that XCD's L2.
+__host__ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nb) {
+  const uint32_t xcd = b & 7u, idx = b >> 3;
+  const uint32_t per = nb >> 3, rem = nb & 7u;
+  return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
 __host__ __device__ inline uint16_t bf16_rne_bits(float v) {
   uint32_t u = __builtin_bit_cast(uint32_t, v);
   if ((u & 0x7FFFFFFFu) > 0x7F800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);
