@@ -93,7 +93,7 @@ struct tsn_batch {
   float2 *tw0 = nullptr, *tw1 = nullptr, *smaps_t = nullptr, *t_fwd = nullptr, *t_inv = nullptr, *grid_buf = nullptr;
   float* apod_t = nullptr;
   uint32_t *pan_il = nullptr, *smp_il = nullptr, *coils_dev = nullptr, *m_dev = nullptr;
-  float2 *x = nullptr, *r = nullptr, *p = nullptr, *p2 = nullptr, *ap = nullptr, *b = nullptr;
+  float2 *x = nullptr, *r = nullptr, *r2 = nullptr, *p = nullptr, *p2 = nullptr, *ap = nullptr, *b = nullptr;
   double *partial = nullptr, *rr_partial = nullptr, *hist = nullptr;
   uint32_t hist_cap = 0;
   int* status = nullptr;
@@ -162,8 +162,7 @@ FftArgs batch_fft_args(const tsn_batch* bt) {
   a.t_stride = static_cast<uint64_t>(bt->cfg.grid0) * bt->cfg.image1 * bt->cmax;
   a.grid_stride = static_cast<uint64_t>(bt->grid) * bt->cmax;
   a.panel_stride = panel_stride(bt);
-  a.partial_stride = bt->cfg.image1;
-  a.rr_stride = bt->vec_blocks;
+  a.partial_stride = 3 * static_cast<uint64_t>(bt->cfg.image1);
   a.hist_stride = bt->hist_cap;
   return a;
 }
@@ -196,21 +195,29 @@ int enqueue_batch_cg(tsn_batch* bt, uint32_t iterations, hipStream_t s) {
                      bt->rr_partial, vs, vb);
   hipLaunchKernelGGL(cg_rho_kernel, dim3(1, S), dim3(kVecThreads), 0, s, bt->rr_partial, vb, bt->hist, 0u, 0,
                      bt->status, vb, bt->hist_cap);
+  // the plan's schedule (plan.hip enqueue_cg) for every problem at once
   float2* p_cur = bt->p;
   float2* p_other = bt->p2;
+  float2* r_cur = bt->r;
+  float2* r_other = bt->r2;
   for (uint32_t it = 0; it < iterations; ++it) {
     FftArgs a = batch_fft_args(bt);
     if (it == 0) {
       a.image_t = p_cur;
     } else {
-      a.cg_r = bt->r;
+      a.cg_r = r_cur;
+      a.cg_r_new = r_other;
+      a.cg_x = bt->x;
+      a.cg_ap = bt->ap;
       a.cg_p_old = p_cur;
       a.cg_p_new = p_other;
-      a.rr_partial = bt->rr_partial;
-      a.rr_count = vb;
+      a.cg_dots = bt->partial;
+      a.dot_count = bt->cfg.image1;
+      a.cg_status = bt->status;
       a.rho_hist = bt->hist;
       a.iter = it;
       std::swap(p_cur, p_other);
+      std::swap(r_cur, r_other);
     }
     TSN_CHECK(fft_forward(a, out_kind_of(bt->cfg.precision), s, S));
     TSN_CHECK(launch_interp_batch(bt->ib_fwd, kind, 3, false, norm, s));
@@ -220,13 +227,12 @@ int enqueue_batch_cg(tsn_batch* bt, uint32_t iterations, hipStream_t s) {
     inv.reg_in = p_cur;
     inv.scale_of = bt->scale_normal;
     inv.dot_partial = bt->partial;
+    inv.dot_r = r_cur;
     TSN_CHECK(fft_inverse(inv, s, S));
-    hipLaunchKernelGGL(cg_update_xr_kernel, dim3(vb, S), dim3(kVecThreads), 0, s, bt->x, bt->r, p_cur, bt->ap,
-                       bt->partial, bt->cfg.image1, bt->hist, it, bt->status, n, bt->rr_partial, vs,
-                       bt->cfg.image1, bt->hist_cap, vb);
   }
-  hipLaunchKernelGGL(cg_rho_kernel, dim3(1, S), dim3(kVecThreads), 0, s, bt->rr_partial, vb, bt->hist, iterations,
-                     1, bt->status, vb, bt->hist_cap);
+  hipLaunchKernelGGL(cg_final_kernel, dim3(vb, S), dim3(kVecThreads), 0, s, bt->x, p_cur, bt->partial,
+                     bt->cfg.image1, bt->hist, iterations - 1, bt->status, n, vs, 3 * bt->cfg.image1,
+                     bt->hist_cap);
   TSN_HIP(hipGetLastError());
   return TSN_OK;
 }
@@ -346,8 +352,8 @@ int tsn_batch_create(const tsn_plan_config* cfg, const tsn_batch_problem* proble
   A(&bt->grid_buf, S * grid * bt->cmax);
   A(&bt->pan_il, S * panel_stride(bt));
   A(&bt->smp_il, S * sample_stride(bt));
-  for (float2** v : {&bt->x, &bt->r, &bt->p, &bt->p2, &bt->ap, &bt->b}) A(v, S * px);
-  A(&bt->partial, S * c.image1);
+  for (float2** v : {&bt->x, &bt->r, &bt->r2, &bt->p, &bt->p2, &bt->ap, &bt->b}) A(v, S * px);
+  A(&bt->partial, S * 3 * static_cast<size_t>(c.image1));
   A(&bt->rr_partial, S * bt->vec_blocks);
   A(&bt->status, S);
   A(&bt->coils_dev, S);

@@ -58,44 +58,44 @@ __global__ __launch_bounds__(kVecThreads) void cg_init_kernel(const float2* __re
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// alpha = rho_k / Re<p, Ap> (0 and status 0 on non-positive or non-finite
-// curvature), x += alpha p, r -= alpha Ap, partial <r, r>.  rho_k is
-// rho_hist[iter]; every block reduces the same <p, Ap> partials itself.
-__global__ __launch_bounds__(kVecThreads) void cg_update_xr_kernel(
-    float2* __restrict__ x, float2* __restrict__ r, const float2* __restrict__ p,
-    const float2* __restrict__ ap, const double* __restrict__ dot_partial, uint32_t dot_count,
-    const double* __restrict__ rho_hist, uint32_t iter, int* __restrict__ status, uint32_t n,
-    double* __restrict__ rr_partial, uint64_t vstride, uint32_t dstride, uint32_t hstride, uint32_t rstride) {
+// The last CG step's update: alpha = rho / <p, Ap> from the inverse pass's
+// partials dots [3][dot_count] (<p, Ap>, Re<r, Ap>, <Ap, Ap>; 0 and status 0
+// on non-positive or non-finite curvature), x += alpha p; block 0 writes
+// rho_hist[iter + 1] = ||r - alpha Ap||^2 (expanded as in the forward pass)
+// and turns rho_hist[0 .. iter + 1] into the residual history sqrt(rho).
+__global__ __launch_bounds__(kVecThreads) void cg_final_kernel(
+    float2* __restrict__ x, const float2* __restrict__ p, const double* __restrict__ dots, uint32_t dot_count,
+    double* __restrict__ rho_hist, uint32_t iter, int* __restrict__ status, uint32_t n, uint64_t vstride,
+    uint32_t dstride, uint32_t hstride) {
   __shared__ double red[kVecThreads];
   const uint64_t z = blockIdx.y;  // problem of a batched launch
   x += z * vstride;
-  r += z * vstride;
   p += z * vstride;
-  ap += z * vstride;
-  dot_partial += z * dstride;
+  dots += z * dstride;
   rho_hist += z * hstride;
   status += z;
-  rr_partial += z * rstride;
-  const double den = sum_partials(dot_partial, dot_count, red);
+  const double pap = sum_partials(dots, dot_count, red);
   __syncthreads();
+  const double rap = sum_partials(dots + dot_count, dot_count, red);
+  __syncthreads();
+  const double apap = sum_partials(dots + 2 * dot_count, dot_count, red);
+  const double rho = rho_hist[iter];
   double alpha = 0.0;
-  if (isfinite(den) && den > 0.0) alpha = rho_hist[iter] / den;
+  if (isfinite(pap) && pap > 0.0) alpha = rho / pap;
   else if (blockIdx.x == 0 && threadIdx.x == 0) *status = 0;
-  double s = 0.0;
   const uint32_t e = blockIdx.x * kVecThreads + threadIdx.x;
   if (e < n) {
-    const float2 pv = p[e], av = ap[e];
-    float2 xv = x[e], rv = r[e];
+    const float2 pv = p[e];
+    float2 xv = x[e];
     xv.x = static_cast<float>(xv.x + alpha * pv.x);
     xv.y = static_cast<float>(xv.y + alpha * pv.y);
-    rv.x = static_cast<float>(rv.x - alpha * av.x);
-    rv.y = static_cast<float>(rv.y - alpha * av.y);
     x[e] = xv;
-    r[e] = rv;
-    s = static_cast<double>(rv.x) * rv.x + static_cast<double>(rv.y) * rv.y;
   }
-  s = block_sum(s, red);
-  if (threadIdx.x == 0) rr_partial[blockIdx.x] = s;
+  if (blockIdx.x != 0) return;
+  __syncthreads();
+  if (threadIdx.x == 0) rho_hist[iter + 1] = fmax(rho - 2.0 * alpha * rap + alpha * alpha * apap, 0.0);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i <= iter + 1; i += blockDim.x) rho_hist[i] = sqrt(fmax(rho_hist[i], 0.0));
 }
 
 // rho_hist[index] = sum of partials (status = 1 when index == 0).  With

@@ -37,13 +37,14 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
   const uint32_t c = chunk * 8 + w;
   const bool active = c < a.coils;
   const WaveTwiddles<R> twp = load_twiddles<R>(a.tw0, lane);
-  // issue every data load first (image or r and p, maps, apodization); the
-  // direction update's beta reduction then overlaps their latency
-  float2 img[R], sm[R], pv[R];
+  // issue every data load first (image, or r, p, Ap and x; maps,
+  // apodization); the reductions of the CG scalars then overlap their latency
+  const bool store_vec = chunk == 0 && w == 0;  // the wave that writes x, r and p_new
+  float2 img[R], sm[R], pv[R], av[R], xv[R];
   float ap[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) {
-    img[j] = pv[j] = sm[j] = make_float2(0.f, 0.f);
+    img[j] = pv[j] = sm[j] = av[j] = xv[j] = make_float2(0.f, 0.f);
     ap[j] = 0.f;
     const uint32_t n0 = lane + 64 * j;
     if (active && n0 < a.n0) {
@@ -51,6 +52,8 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
       if (a.cg_r) {
         img[j] = a.cg_r[pix];
         pv[j] = a.cg_p_old[pix];
+        av[j] = a.cg_ap[pix];
+        if (store_vec) xv[j] = a.cg_x[pix];
       } else {
         img[j] = a.image_t[pix];
       }
@@ -59,23 +62,31 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
     }
   }
   if (a.cg_r) {
-    // every block reduces the same partials in the same order: same beta
-    double s = 0.0;
-    for (uint32_t i = threadIdx.x; i < a.rr_count; i += blockDim.x) s += a.rr_partial[i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 256; off > 0; off >>= 1) {
-      if (static_cast<int>(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
-      __syncthreads();
-    }
-    const double rho_new = red[0];
-    const double beta = rho_new / fmax(a.rho_hist[a.iter - 1], 1e-30);
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.rho_hist[a.iter] = rho_new;
+    // the previous step's scalars: every block reduces the same partials in the same order
+    const double pap = sum_partials512(a.cg_dots, a.dot_count, red);
+    const double rap = sum_partials512(a.cg_dots + a.dot_count, a.dot_count, red);
+    const double apap = sum_partials512(a.cg_dots + 2 * a.dot_count, a.dot_count, red);
+    const double rho_old = a.rho_hist[a.iter - 1];
+    const bool first_block = blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0;
+    double alpha = 0.0;
+    if (isfinite(pap) && pap > 0.0) alpha = rho_old / pap;
+    else if (first_block) *a.cg_status = 0;
+    const double rho_new = fmax(rho_old - 2.0 * alpha * rap + alpha * alpha * apap, 0.0);
+    const double beta = rho_new / fmax(rho_old, 1e-30);
+    if (first_block) a.rho_hist[a.iter] = rho_new;
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const uint32_t n0 = lane + 64 * j;
-      img[j] = make_float2(static_cast<float>(img[j].x + beta * pv[j].x), static_cast<float>(img[j].y + beta * pv[j].y));
-      if (active && n0 < a.n0 && chunk == 0 && w == 0) a.cg_p_new[static_cast<size_t>(n1) * a.n0 + n0] = img[j];
+      const float2 r_new = make_float2(static_cast<float>(img[j].x - alpha * av[j].x),
+                                       static_cast<float>(img[j].y - alpha * av[j].y));
+      img[j] = make_float2(static_cast<float>(r_new.x + beta * pv[j].x), static_cast<float>(r_new.y + beta * pv[j].y));
+      if (active && n0 < a.n0 && store_vec) {
+        const size_t pix = static_cast<size_t>(n1) * a.n0 + n0;
+        a.cg_x[pix] = make_float2(static_cast<float>(xv[j].x + alpha * pv[j].x),
+                                  static_cast<float>(xv[j].y + alpha * pv[j].y));
+        a.cg_r_new[pix] = r_new;
+        a.cg_p_new[pix] = img[j];
+      }
     }
   }
   float2 x[R];
@@ -233,7 +244,7 @@ __global__ __launch_bounds__(512) void inv_axis0_kernel(FftArgs args) {
     }
     __syncthreads();
   }
-  double part = 0.0;
+  double part = 0.0, part_r = 0.0, part_v = 0.0;
   if (threadIdx.x < a.n0) {
     const size_t pix = static_cast<size_t>(n1) * a.n0 + threadIdx.x;
     const float s = a.apod_t[pix] * a.out_scale;
@@ -244,16 +255,24 @@ __global__ __launch_bounds__(512) void inv_axis0_kernel(FftArgs args) {
       v.y += a.lambda * p.y;
       part = static_cast<double>(p.x) * v.x + static_cast<double>(p.y) * v.y;
     }
+    if (a.dot_r) {
+      const float2 r = a.dot_r[pix];
+      part_r = static_cast<double>(r.x) * v.x + static_cast<double>(r.y) * v.y;
+      part_v = static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y;
+    }
     a.image_out[pix] = v;
   }
   if (a.dot_partial) {
-    red[threadIdx.x] = part;
-    __syncthreads();
-    for (int off = 256; off > 0; off >>= 1) {
-      if (static_cast<int>(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
-      __syncthreads();
+    const double d = block_sum512(part, red);
+    if (threadIdx.x == 0) a.dot_partial[n1] = d;
+    if (a.dot_r) {
+      const double dr = block_sum512(part_r, red);
+      const double dv = block_sum512(part_v, red);
+      if (threadIdx.x == 0) {
+        a.dot_partial[a.n1 + n1] = dr;
+        a.dot_partial[2 * a.n1 + n1] = dv;
+      }
     }
-    if (threadIdx.x == 0) a.dot_partial[n1] = red[0];
   }
 }
 

@@ -27,17 +27,28 @@ struct FftArgs {
   const float2* reg_in;    // [n1][n0] or null
   float lambda;
   float out_scale;
-  double* dot_partial;     // [n1] or null
-  // CG direction update fused into the forward first axis (cg_r null: the
-  // input is image_t).  Input p_new = r + beta p_old, beta = rho_new /
-  // max(rho_old, 1e-30), rho_new = fixed-order sum of rr_partial; rho values
-  // live in rho_hist (rho_hist[iter - 1] read, rho_hist[iter] written by
-  // block (0, 0)); coil chunk 0 stores p_new.
+  double* dot_partial;     // [n1] partials of Re<reg_in, out>, or null
+  const float2* dot_r;     // with dot_partial: also Re<dot_r, out> at [n1 + n1] and <out, out> at [2 n1 + n1]
+  // The previous CG step's update fused into the forward first axis (cg_r
+  // null: the input is image_t).  Every block reduces the partials cg_dots
+  // [3][dot_count] (<p, Ap>, Re<r, Ap>, <Ap, Ap>) in the same order: alpha =
+  // rho_old / <p, Ap> (0 and *cg_status = 0 on non-positive or non-finite
+  // curvature), rho_new = rho_old - 2 alpha Re<r, Ap> + alpha^2 <Ap, Ap>
+  // (= ||r - alpha Ap||^2), beta = rho_new / max(rho_old, 1e-30); then
+  // x += alpha p_old, r_new = r - alpha Ap, and the input is p_new = r_new +
+  // beta p_old.  rho_hist[iter - 1] is read and rho_hist[iter] written by
+  // block (0, 0); coil chunk 0's first wave stores x (in place: it alone reads
+  // it), r_new and p_new (other buffers than r and p_old, which every chunk
+  // and wave of the column reads).
   const float2* cg_r;
+  float2* cg_r_new;
+  float2* cg_x;
+  const float2* cg_ap;
   const float2* cg_p_old;
   float2* cg_p_new;
-  const double* rr_partial;
-  uint32_t rr_count;
+  const double* cg_dots;
+  uint32_t dot_count;
+  int* cg_status;
   double* rho_hist;
   uint32_t iter;
   // Batched launches (blockIdx.z = problem z): every per-problem buffer sits
@@ -45,15 +56,35 @@ struct FftArgs {
   // shared.  A single problem leaves them 0.
   const uint32_t* coils_of;  // [problems] or null
   const float2* scale_of;    // [problems] {out_scale, lambda} or null (inverse pass)
-  uint64_t image_stride;     // image_t, image_out, reg_in, cg_r, cg_p_old, cg_p_new
+  uint64_t image_stride;     // image_t, image_out, reg_in, dot_r, cg_r, cg_r_new, cg_x, cg_ap, cg_p_old, cg_p_new
   uint64_t smaps_stride;     // smaps_t
   uint64_t t_stride;         // t_fwd, t_inv
   uint64_t grid_stride;      // grid, panel32
   uint64_t panel_stride;     // panel_il (uint32 words)
-  uint64_t partial_stride;   // dot_partial
-  uint64_t rr_stride;        // rr_partial
-  uint64_t hist_stride;      // rho_hist
+  uint64_t partial_stride;   // dot_partial, cg_dots
+  uint64_t hist_stride;      // rho_hist (cg_status: one int per problem)
 };
+
+// Sum of v over a 512-thread block through red[512], in a fixed order; every
+// thread gets the sum.  Ends with a barrier, so red can be reused at once.
+__device__ __forceinline__ double block_sum512(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 256; off > 0; off >>= 1) {
+    if (static_cast<int>(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  const double s = red[0];
+  __syncthreads();
+  return s;
+}
+
+// Fixed-order sum of count partials (512-thread block).
+__device__ __forceinline__ double sum_partials512(const double* __restrict__ p, uint32_t count, double* red) {
+  double s = 0.0;
+  for (uint32_t i = threadIdx.x; i < count; i += 512) s += p[i];
+  return block_sum512(s, red);
+}
 
 // The arguments of problem z of a batched launch.
 __device__ __forceinline__ FftArgs problem_view(const FftArgs& a, uint32_t z) {
@@ -68,7 +99,11 @@ __device__ __forceinline__ FftArgs problem_view(const FftArgs& a, uint32_t z) {
   v.image_t = a.image_t ? a.image_t + zi * a.image_stride : nullptr;
   v.image_out = a.image_out ? a.image_out + zi * a.image_stride : nullptr;
   v.reg_in = a.reg_in ? a.reg_in + zi * a.image_stride : nullptr;
+  v.dot_r = a.dot_r ? a.dot_r + zi * a.image_stride : nullptr;
   v.cg_r = a.cg_r ? a.cg_r + zi * a.image_stride : nullptr;
+  v.cg_r_new = a.cg_r_new ? a.cg_r_new + zi * a.image_stride : nullptr;
+  v.cg_x = a.cg_x ? a.cg_x + zi * a.image_stride : nullptr;
+  v.cg_ap = a.cg_ap ? a.cg_ap + zi * a.image_stride : nullptr;
   v.cg_p_old = a.cg_p_old ? a.cg_p_old + zi * a.image_stride : nullptr;
   v.cg_p_new = a.cg_p_new ? a.cg_p_new + zi * a.image_stride : nullptr;
   v.smaps_t = a.smaps_t + zi * a.smaps_stride;
@@ -78,7 +113,8 @@ __device__ __forceinline__ FftArgs problem_view(const FftArgs& a, uint32_t z) {
   v.panel32 = a.panel32 ? a.panel32 + zi * a.grid_stride : nullptr;
   v.panel_il = a.panel_il ? a.panel_il + zi * a.panel_stride : nullptr;
   v.dot_partial = a.dot_partial ? a.dot_partial + zi * a.partial_stride : nullptr;
-  v.rr_partial = a.rr_partial ? a.rr_partial + zi * a.rr_stride : nullptr;
+  v.cg_dots = a.cg_dots ? a.cg_dots + zi * a.partial_stride : nullptr;
+  v.cg_status = a.cg_status ? a.cg_status + zi : nullptr;
   v.rho_hist = a.rho_hist ? a.rho_hist + zi * a.hist_stride : nullptr;
   return v;
 }

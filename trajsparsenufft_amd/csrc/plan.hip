@@ -10,8 +10,8 @@
 //   CG update order, curvature flag (alpha = 0)   :56-180, 241-268
 // Differences by design: float64 CG scalars (rho, <p,Ap>, alpha, beta, history),
 // reduced in fixed order by every block that needs them, so an iteration is
-// seven kernels (the direction update rides in the first forward FFT pass,
-// alpha in the x/r update);
+// six kernels (the x, r and direction updates ride in the first forward FFT
+// pass, with ||r - alpha Ap||^2 expanded from three inverse-pass partials);
 // image vectors kept column-major [n1][n0] inside the plan; SENSE combine,
 // Tikhonov term and the <p, Ap> partial fused into the last inverse FFT pass;
 // the forward interpolation writes the adjoint's FP16 hi/res sample panels
@@ -51,8 +51,10 @@ struct tsn_plan {
   uint32_t *pan_il = nullptr, *smp_il = nullptr;  // interleaved hi | res << 16 panels
   float2* y_rows = nullptr;                         // measurement rows [M][coils] (data consistency)
   float2 *pan32 = nullptr, *smp32 = nullptr, *grid = nullptr;
-  float2 *x = nullptr, *r = nullptr, *p = nullptr, *p2 = nullptr, *ap = nullptr, *b = nullptr, *tmp = nullptr;
-  double *partial = nullptr, *rr_partial = nullptr;  // <p, Ap> per image column; <r, r> per vector block
+  float2 *x = nullptr, *r = nullptr, *r2 = nullptr, *p = nullptr, *p2 = nullptr, *ap = nullptr, *b = nullptr,
+         *tmp = nullptr;
+  // [3][image1] partials of <p, Ap>, Re<r, Ap>, <Ap, Ap> per image column; <r, r> per vector block
+  double *partial = nullptr, *rr_partial = nullptr;
   int* status = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -141,13 +143,14 @@ int op_adjoint_grid(tsn_plan* pl, hipStream_t s) {
 
 // grid -> image (column-major) with combine, scale, regulariser, dot partial
 int op_adjoint_image(tsn_plan* pl, float2* out_t, const float2* reg, float lambda, float out_scale,
-                     double* dot_partial, hipStream_t s) {
+                     double* dot_partial, hipStream_t s, const float2* dot_r = nullptr) {
   FftArgs a = fft_args(pl);
   a.image_out = out_t;
   a.reg_in = reg;
   a.lambda = lambda;
   a.out_scale = out_scale * grid_norm(pl);
   a.dot_partial = dot_partial;
+  a.dot_r = dot_r;
   return fft_inverse(a, s);
 }
 
@@ -195,34 +198,41 @@ int enqueue_cg(tsn_plan* pl, const float2* measurement, uint32_t iterations, flo
   hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kVecThreads), 0, s, pl->rr_partial, pl->vec_blocks, history,
                      0u, 0, pl->status, 0u, 0u);
   // rho_k lives in history[k] until the end.  Iteration k: the first forward
-  // FFT pass forms p_k = r + beta p_{k-1} (k > 0, ping-pong buffers) and
-  // rho_k; the last inverse pass forms <p_k, A p_k>; one kernel applies alpha.
+  // FFT pass applies step k - 1 (alpha from the partials of the last inverse
+  // pass: x += alpha p, r -= alpha Ap, rho_k) and forms p_k = r_k + beta
+  // p_{k-1} (k > 0; r and p in ping-pong buffers); the last inverse pass forms
+  // A p_k and the partials of <p_k, A p_k>, Re<r_k, A p_k>, <A p_k, A p_k>.
+  // One kernel applies the last step.  Six kernels per iteration.
   float2* p_cur = pl->p;
   float2* p_other = pl->p2;
+  float2* r_cur = pl->r;
+  float2* r_other = pl->r2;
   for (uint32_t it = 0; it < iterations; ++it) {
     FftArgs a = fft_args(pl);
     if (it == 0) {
       a.image_t = p_cur;
     } else {
-      a.cg_r = pl->r;
+      a.cg_r = r_cur;
+      a.cg_r_new = r_other;
+      a.cg_x = pl->x;
+      a.cg_ap = pl->ap;
       a.cg_p_old = p_cur;
       a.cg_p_new = p_other;
-      a.rr_partial = pl->rr_partial;
-      a.rr_count = pl->vec_blocks;
+      a.cg_dots = pl->partial;
+      a.dot_count = pl->cfg.image1;
+      a.cg_status = pl->status;
       a.rho_hist = history;
       a.iter = it;
       std::swap(p_cur, p_other);
+      std::swap(r_cur, r_other);
     }
     TSN_CHECK(fft_forward(a, fft_out_kind(pl), s));
     TSN_CHECK(op_forward_samples(pl, nullptr, s));
     TSN_CHECK(op_adjoint_grid(pl, s));
-    TSN_CHECK(op_adjoint_image(pl, pl->ap, p_cur, scaled_lambda, inv_scale_sq, pl->partial, s));
-    hipLaunchKernelGGL(cg_update_xr_kernel, dim3(pl->vec_blocks), dim3(kVecThreads), 0, s, pl->x, pl->r,
-                       p_cur, pl->ap, pl->partial, pl->cfg.image1, history, it, pl->status, n, pl->rr_partial,
-                       uint64_t{0}, 0u, 0u, 0u);
+    TSN_CHECK(op_adjoint_image(pl, pl->ap, p_cur, scaled_lambda, inv_scale_sq, pl->partial, s, r_cur));
   }
-  hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kVecThreads), 0, s, pl->rr_partial, pl->vec_blocks, history,
-                     iterations, 1, pl->status, 0u, 0u);
+  hipLaunchKernelGGL(cg_final_kernel, dim3(pl->vec_blocks), dim3(kVecThreads), 0, s, pl->x, p_cur, pl->partial,
+                     pl->cfg.image1, history, iterations - 1, pl->status, n, uint64_t{0}, 0u, 0u);
   TSN_CHECK(transpose_image(pl->x, pl->cfg.image1, pl->cfg.image0, 1.0f, x_out, s));
   hipLaunchKernelGGL(copy_status_kernel, dim3(1), dim3(64), 0, s, pl->status, status_out);
   TSN_HIP(hipGetLastError());
@@ -307,8 +317,8 @@ int tsn_plan_create(const tsn_plan_config* cfg, const tsn_direction* fwd, const 
     A(&pl->smp_il, static_cast<size_t>(pl->blocks) * pl->m * kN);
     A(&pl->y_rows, static_cast<size_t>(pl->m) * c.coils);
   }
-  for (float2** v : {&pl->x, &pl->r, &pl->p, &pl->p2, &pl->ap, &pl->b, &pl->tmp}) A(v, px);
-  A(&pl->partial, std::max<size_t>(pl->vec_blocks, c.image1));
+  for (float2** v : {&pl->x, &pl->r, &pl->r2, &pl->p, &pl->p2, &pl->ap, &pl->b, &pl->tmp}) A(v, px);
+  A(&pl->partial, 3 * static_cast<size_t>(c.image1));
   A(&pl->rr_partial, pl->vec_blocks);
   A(&pl->status, 1);
   if (rc == TSN_OK) {
