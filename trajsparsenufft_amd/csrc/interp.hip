@@ -387,13 +387,15 @@ struct PipeDepth {
 // issue order, so the multiply's wait leaves D tiles of loads in flight.
 // Tile i lives in register slot i % U of both rings (U = max(D + 1, 2), the
 // loop unrolled U times so every slot index is a constant).  Indices past the
-// end are clamped to the last tile (valid, unused loads).  GROUPS: the tiles
-// are those of the consecutive groups group, group + 1, ... (< group_end); the
-// last tile of each runs the epilogue with the group's rows (loaded one group
-// ahead) and resets the accumulators.  Otherwise everything accumulates.
-template <int KIND, int NB, bool IL, bool GROUPS, int OM, bool CONJ, bool ACC>
+// end are clamped to the last tile (valid, unused loads).  The tiles are
+// those of the consecutive groups group, group + 1, ... (< group_end); with
+// last_mask 1 the last tile of each runs the epilogue with the group's rows
+// (loaded one group ahead) and resets the accumulators; with 0 (a chunk of a
+// long group) everything accumulates.  One loop for both keeps the kernel's
+// register count at that of the regular units.
+template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC>
 __device__ __forceinline__ void stream_tiles(const InterpArgs& a, const TileRsrc<NB>& r, int first, int end,
-                                             int group, int group_end, float4_t (&acc_re)[NB],
+                                             int group, int group_end, uint32_t last_mask, float4_t (&acc_re)[NB],
                                              float4_t (&acc_im)[NB], int lane, const uint32_t* stage) {
   constexpr int D = PipeDepth<KIND, NB, IL>::value;
   constexpr int U = D + 1 > 2 ? D + 1 : 2;
@@ -407,8 +409,8 @@ __device__ __forceinline__ void stream_tiles(const InterpArgs& a, const TileRsrc
   const int last = end - 1;
   auto tile = [&](int i) { return min(first + i, last); };
   auto cols_of = [&](int i) { return staged_cols(stage, tile(i) - first, lane); };
-  int4 rows = make_int4(-1, -1, -1, -1);
-  if constexpr (GROUPS) rows = load_rows(r, g, lane);
+  last_mask = __builtin_amdgcn_readfirstlane(last_mask);
+  int4 rows = load_rows(r, g, lane);
   TileCols cols[U];
   TileOps<NB, IL> ops[U];
   // The scheduler must keep the issue order: the wait counter is in order, and
@@ -433,25 +435,23 @@ __device__ __forceinline__ void stream_tiles(const InterpArgs& a, const TileRsrc
       fetch_ops<KIND, NB, IL>(r, tile(i + D), cols[(j + D) % U], lane, ops[(j + D) % U]);
       if (i < n) {
         mma_tile<KIND, NB, IL>(ops[j], acc_re, acc_im);
-        if constexpr (GROUPS) {
-          if (__builtin_amdgcn_readfirstlane(ops[j].last)) {
+        if (__builtin_amdgcn_readfirstlane(ops[j].last) & last_mask) {
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-              interp_epilogue<OM, CONJ, ACC>(a, rows, b, acc_re[b], acc_im[b], lane);
-              acc_re[b] = float4_t{0.f, 0.f, 0.f, 0.f};
-              acc_im[b] = float4_t{0.f, 0.f, 0.f, 0.f};
-            }
-            g = min(g + 1, g_last);
-            rows = load_rows(r, g, lane);
+          for (int b = 0; b < NB; ++b) {
+            interp_epilogue<OM, CONJ, ACC>(a, rows, b, acc_re[b], acc_im[b], lane);
+            acc_re[b] = float4_t{0.f, 0.f, 0.f, 0.f};
+            acc_im[b] = float4_t{0.f, 0.f, 0.f, 0.f};
           }
+          g = min(g + 1, g_last);
+          rows = load_rows(r, g, lane);
         }
       }
     }
   }
 }
 
-// One chunk of a long group: stream its tiles, publish the partial sums, and
-// if this wave is the group's last arrival, add every chunk's slot in chunk
+// After a chunk of a long group has streamed its tiles: publish the partial
+// sums, and if this wave is the group's last arrival, add every chunk's slot in chunk
 // order and write the group's rows.  Slots are written and read with
 // agent-scope (device-coherent) relaxed atomic stores and loads, which reach
 // past the per-XCD L2 without a cache flush; the stores complete (vmcnt 0)
@@ -459,13 +459,9 @@ __device__ __forceinline__ void stream_tiles(const InterpArgs& a, const TileRsrc
 // shows every chunk done.  A full agent-scope fence here would write back the
 // whole L2 per chunk (measured: the launch went from 29 to 230 us).  The
 // last wave resets the counter for the next launch.  No wave waits on another.
-template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC>
-__device__ __forceinline__ void long_chunk(const InterpArgs& a, const ChunkItem& c, float4_t (&acc_re)[NB],
-                                           float4_t (&acc_im)[NB], int lane, uint32_t* stage) {
-  const TileRsrc<NB> r = make_tile_rsrc<NB, IL>(a);
-  stage_cols(r, c.tile_begin, c.tile_end - c.tile_begin, stage, lane);
-  stream_tiles<KIND, NB, IL, false, OM, CONJ, ACC>(a, r, c.tile_begin, c.tile_end, c.group, c.group + 1, acc_re,
-                                                   acc_im, lane, stage);
+template <int OM, bool CONJ, bool ACC, int NB>
+__device__ __forceinline__ void chunk_finish(const InterpArgs& a, const ChunkItem& c, float4_t (&acc_re)[NB],
+                                             float4_t (&acc_im)[NB], int lane) {
   // slot layout: [coil block][re, im][lane][4] floats
   float* mine = a.partials + static_cast<size_t>(c.slot0 + c.chunk) * kSlotFloats;
 #pragma unroll
@@ -488,50 +484,48 @@ __device__ __forceinline__ void long_chunk(const InterpArgs& a, const ChunkItem&
   if (lane == 0) __hip_atomic_store(a.counters + c.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const float* first = a.partials + static_cast<size_t>(c.slot0) * kSlotFloats;
   const int4 rows = group_rows_of(a, static_cast<uint32_t>(c.group), lane);
+  const uint32_t n = c.n_chunks;
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    float4_t re, im;
-    for (uint32_t q = 0; q < c.n_chunks; ++q) {
-      const float* p = first + static_cast<size_t>(q) * kSlotFloats;
-      float4_t pr, pi;
+    float4_t re = float4_t{0.f, 0.f, 0.f, 0.f}, im = re;
+    // slots are read kBatch at a time (all loads issued before the first
+    // add), then added in chunk order: ceil(n / kBatch) memory round trips
+    constexpr uint32_t kBatch = 4;
+    for (uint32_t q0 = 0; q0 < n; q0 += kBatch) {
+      float4_t pr[kBatch], pi[kBatch];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        pr[k] = __hip_atomic_load(p + ((b * 2 + 0) * 64 + lane) * 4 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        pi[k] = __hip_atomic_load(p + ((b * 2 + 1) * 64 + lane) * 4 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (uint32_t j = 0; j < kBatch; ++j) {
+        const float* p = first + static_cast<size_t>(min(q0 + j, n - 1)) * kSlotFloats;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          pr[j][k] = __hip_atomic_load(p + ((b * 2 + 0) * 64 + lane) * 4 + k, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+          pi[j][k] = __hip_atomic_load(p + ((b * 2 + 1) * 64 + lane) * 4 + k, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
-      if (q == 0) {
-        re = pr;
-        im = pi;
-      } else {
-        re += pr;
-        im += pi;
+#pragma unroll
+      for (uint32_t j = 0; j < kBatch; ++j) {
+        if (q0 + j >= n) break;
+        if (q0 + j == 0) {
+          re = pr[j];
+          im = pi[j];
+        } else {
+          re += pr[j];
+          im += pi[j];
+        }
       }
     }
     interp_epilogue<OM, CONJ, ACC>(a, rows, b, re, im, lane);
   }
 }
 
-// A regular unit: one wave streams its whole groups.
-template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC>
-__device__ __forceinline__ void regular_unit(const InterpArgs& a, const WorkItem& u, float4_t (&acc_re)[NB],
-                                             float4_t (&acc_im)[NB], int lane, uint32_t* stage) {
-  if (u.tile_begin == u.tile_end) {  // empty group: zero rows
-    const int4 rows = group_rows_of(a, static_cast<uint32_t>(u.group), lane);
-#pragma unroll
-    for (int b = 0; b < NB; ++b) interp_epilogue<OM, CONJ, ACC>(a, rows, b, acc_re[b], acc_im[b], lane);
-    return;
-  }
-  const TileRsrc<NB> r = make_tile_rsrc<NB, IL>(a);
-  stage_cols(r, u.tile_begin, u.tile_end - u.tile_begin, stage, lane);
-  stream_tiles<KIND, NB, IL, true, OM, CONJ, ACC>(a, r, u.tile_begin, u.tile_end, u.group, u.group_end, acc_re,
-                                                  acc_im, lane, stage);
-}
-
 // One wave per block, one entry of the launch order each (a regular unit, a
 // run of whole groups, or a chunk of a long group), the entries of one XCD
 // contiguous in pack order.  One-wave blocks let the dispatcher start a new
 // entry whenever a wave slot frees, so uneven entries balance themselves
-// across the CUs.
+// across the CUs.  Units and chunks share one tile loop (one inlined copy
+// keeps the register count of the regular units).
 template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC>
 __global__ __launch_bounds__(64) void interp_tiles_kernel(InterpArgs a) {
   __shared__ uint32_t stage[kStageWords];
@@ -543,27 +537,33 @@ __global__ __launch_bounds__(64) void interp_tiles_kernel(InterpArgs a) {
     acc_im[b] = float4_t{0.f, 0.f, 0.f, 0.f};
   }
   const uint32_t entry = a.order[xcd_remap(blockIdx.x, gridDim.x)];
-  if (entry & kChunkTag) {
-    const ChunkItem c = a.chunks[entry & ~kChunkTag];
-    if constexpr (NB == 1 && IL) {
-      if (a.problems) {
-        long_chunk<KIND, NB, IL, OM, CONJ, ACC>(problem_args(a, c.problem), c, acc_re, acc_im, lane, stage);
-        return;
-      }
-    }
-    long_chunk<KIND, NB, IL, OM, CONJ, ACC>(a, c, acc_re, acc_im, lane, stage);
+  const bool is_chunk = (entry & kChunkTag) != 0;
+  ChunkItem c{};
+  WorkItem u;
+  uint32_t problem = 0;
+  if (is_chunk) {
+    c = a.chunks[entry & ~kChunkTag];
+    u = {c.group, c.group + 1, c.tile_begin, c.tile_end};
+    problem = c.problem;
+  } else {
+    u = a.items[entry];
+    if (a.problems) problem = a.item_problem[entry];
+  }
+  InterpArgs v = a;
+  if constexpr (NB == 1 && IL) {
+    if (a.problems) v = problem_args(a, problem);
+  }
+  if (u.tile_begin == u.tile_end) {  // empty group: zero rows
+    const int4 rows = group_rows_of(v, static_cast<uint32_t>(u.group), lane);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) interp_epilogue<OM, CONJ, ACC>(v, rows, b, acc_re[b], acc_im[b], lane);
     return;
   }
-  const uint32_t item_id = entry;
-  const WorkItem u = a.items[item_id];
-  if constexpr (NB == 1 && IL) {
-    if (a.problems) {
-      regular_unit<KIND, NB, IL, OM, CONJ, ACC>(problem_args(a, a.item_problem[item_id]), u, acc_re, acc_im,
-                                                lane, stage);
-      return;
-    }
-  }
-  regular_unit<KIND, NB, IL, OM, CONJ, ACC>(a, u, acc_re, acc_im, lane, stage);
+  const TileRsrc<NB> r = make_tile_rsrc<NB, IL>(v);
+  stage_cols(r, u.tile_begin, u.tile_end - u.tile_begin, stage, lane);
+  stream_tiles<KIND, NB, IL, OM, CONJ, ACC>(v, r, u.tile_begin, u.tile_end, u.group, u.group_end, is_chunk ? 0u : 1u,
+                                            acc_re, acc_im, lane, stage);
+  if (is_chunk) chunk_finish<OM, CONJ, ACC, NB>(v, c, acc_re, acc_im, lane);
 }
 
 
