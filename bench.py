@@ -338,10 +338,12 @@ def interp_roofline(torch, packs, steps, warmup, log):
 
 
 def roofline_only(args) -> int:
-    """Interpolation launches plus the calibration copy, for counter runs."""
+    """Interpolation launches, the calibration copy and three split-FP16
+    reconstructions (every FFT and CG kernel), for counter runs."""
 
     import torch
     from trajsparsenufft_amd import pack, synth
+    from trajsparsenufft_amd.recon import build_plan
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     traj = synth.radial(SPOKES, READOUT, golden=True)
     packs = pack.prepare_trajectory(traj, (IMAGE, IMAGE), (GRID, GRID))
@@ -350,6 +352,13 @@ def roofline_only(args) -> int:
     dst = torch.empty_like(src)
     for _ in range(args.steps):
         dst.copy_(src)
+    plan = build_plan(packs, torch.from_numpy(synth.coil_maps(COILS, IMAGE)).cuda(), "split16")
+    rng = np.random.default_rng(3)
+    m = traj.shape[0]
+    y = torch.from_numpy((rng.standard_normal((COILS, m)) + 1j * rng.standard_normal((COILS, m))).astype(np.complex64)).cuda()
+    scale = plan.operator_scale(y)
+    for _ in range(3):
+        plan.cg(y, CG_ITERS, scale, LAMBDA)
     torch.cuda.synchronize()
     print(json.dumps({k: roof[k] for k in ("forward", "adjoint")}), flush=True)
     return 0
