@@ -90,6 +90,16 @@ struct InterpProblem {
   uint32_t* out_il;     // this block's split output panel [n_out][16]
 };
 
+// Global-space view of a plain pointer.  The kernel's loads and stores after
+// the first tile go through it: a generic (flat) access counts on both the
+// vector-memory and the LDS wait counters, and while one is outstanding the
+// compiler drains every later wait to zero, which stalls the tile pipeline at
+// each epilogue.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* glb(T* p) {
+  return (__attribute__((address_space(1))) T*)(p);
+}
+
 __device__ __forceinline__ InterpArgs problem_args(const InterpArgs& a, uint32_t vp) {
   const InterpProblem& q = a.problems[vp];
   InterpArgs v = a;
@@ -140,10 +150,10 @@ __device__ __forceinline__ float dc_residual(float pred, float y, float scale, f
 
 __device__ __forceinline__ float dc_apply(const InterpArgs& a, float v, uint32_t coil, int row, int part) {
   asm volatile("" : "+v"(v));  // v is final here, as if loaded from memory
-  const float y = coil < a.coils ? a.dc_meas[static_cast<size_t>(row) * a.dc_row_stride +
+  const float y = coil < a.coils ? glb(a.dc_meas)[static_cast<size_t>(row) * a.dc_row_stride +
                                              static_cast<size_t>(coil) * a.dc_coil_stride + part]
                                  : 0.0f;
-  const float w = a.dc_density ? a.dc_density[row] : 1.0f;
+  const float w = a.dc_density ? glb(a.dc_density)[row] : 1.0f;
   return dc_residual(v, y, a.dc_scale, w);
 }
 
@@ -166,21 +176,21 @@ __device__ __forceinline__ void interp_epilogue(const InterpArgs& a, const int4&
     if (row < 0) continue;
     if constexpr (OM == kOutSplitBF16IL || OM == kOutSplit16IL) {
       if (a.dc_meas) v = dc_apply(a, v, coil, row, part);
-      a.out_il[(static_cast<size_t>(b) * a.n_out + row) * kN + n] = split_word<OM == kOutSplitBF16IL>(v);
+      glb(a.out_il)[(static_cast<size_t>(b) * a.n_out + row) * kN + n] = split_word<OM == kOutSplitBF16IL>(v);
     } else if constexpr (OM == kOutSplit16) {
       asm volatile("" : "+v"(v));  // split the FP32 value (see split_word)
       const _Float16 hi = static_cast<_Float16>(v);
       const _Float16 res = static_cast<_Float16>(v - static_cast<float>(hi));
       const size_t o = (static_cast<size_t>(b) * a.n_out + row) * kN + n;
-      a.out_hi[o] = hi;
-      a.out_res[o] = res;
+      glb(a.out_hi)[o] = hi;
+      glb(a.out_res)[o] = res;
     } else {
       if (coil >= a.coils) continue;
       size_t o;
       if constexpr (OM == kOutCoilMajor) o = (static_cast<size_t>(coil) * a.n_out + row) * 2 + part;
       else o = (static_cast<size_t>(row) * a.coils + coil) * 2 + part;
-      if constexpr (ACC) v += a.out_f32[o];
-      a.out_f32[o] = v;
+      if constexpr (ACC) v += glb(a.out_f32)[o];
+      glb(a.out_f32)[o] = v;
     }
   }
 }
@@ -234,7 +244,8 @@ __device__ __forceinline__ int4 load_rows(const TileRsrc<NB>& r, int g, int lane
 }
 
 __device__ __forceinline__ int4 group_rows_of(const InterpArgs& a, uint32_t group, int lane) {
-  return *reinterpret_cast<const int4*>(a.group_rows + static_cast<size_t>(group) * 16 + (lane >> 4) * 4);
+  const auto* p = glb(a.group_rows + static_cast<size_t>(group) * 16 + (lane >> 4) * 4);
+  return make_int4(p[0], p[1], p[2], p[3]);
 }
 
 // Stage 1 of a tile: the lane's 8 device column words (tile columns
@@ -451,28 +462,34 @@ __device__ __forceinline__ void stream_tiles(const InterpArgs& a, const TileRsrc
 }
 
 // After a chunk of a long group has streamed its tiles: publish the partial
-// sums, and if this wave is the group's last arrival, add every chunk's slot in chunk
-// order and write the group's rows.  Slots are written and read with
-// agent-scope (device-coherent) relaxed atomic stores and loads, which reach
-// past the per-XCD L2 without a cache flush; the stores complete (vmcnt 0)
-// before the arrival is counted, and the slots are read only after the count
-// shows every chunk done.  A full agent-scope fence here would write back the
-// whole L2 per chunk (measured: the launch went from 29 to 230 us).  The
-// last wave resets the counter for the next launch.  No wave waits on another.
+// sums, and if this wave is the group's last arrival, add every chunk's slot in
+// chunk order and write the group's rows.  Slots are written and read with
+// device-scope (sc1) 16-byte buffer accesses, the access a relaxed agent-scope
+// atomic of each word compiles to: they reach past the per-XCD L2 without a
+// cache flush.  The stores complete (vmcnt 0) before the arrival is counted,
+// and the slots are read only after the count shows every chunk done.  A full
+// agent-scope fence here would write back the whole L2 per chunk (measured:
+// the launch went from 29 to 230 us).  The last wave resets the counter for
+// the next launch.  No wave waits on another.
+constexpr int kCpolDevice = 16;  // cache policy bits of a buffer access: sc1 (device scope)
+constexpr uint32_t kSlotBatch = 8;  // slots loaded before the first add (one round trip for <= 8 chunks)
+// Slot byte offsets are 32-bit buffer offsets: at most 2^31 bytes of slots.
+constexpr uint32_t kMaxSlots = (1u << 31) / (kSlotFloats * 4);
+
 template <int OM, bool CONJ, bool ACC, int NB>
 __device__ __forceinline__ void chunk_finish(const InterpArgs& a, const ChunkItem& c, float4_t (&acc_re)[NB],
                                              float4_t (&acc_im)[NB], int lane) {
-  // slot layout: [coil block][re, im][lane][4] floats
-  float* mine = a.partials + static_cast<size_t>(c.slot0 + c.chunk) * kSlotFloats;
+  // slot layout: [coil block][re, im][lane][4] floats; slot offsets stay below
+  // 2^32 bytes (kSlotFloats * 4 bytes per chunk)
+  const rsrc_t slots = make_rsrc(a.partials, uint64_t{1} << 32);
+  const int mine = static_cast<int>((c.slot0 + c.chunk) * kSlotFloats * 4);
 #pragma unroll
-  for (int b = 0; b < NB; ++b)
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      __hip_atomic_store(mine + ((b * 2 + 0) * 64 + lane) * 4 + k, acc_re[b][k], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(mine + ((b * 2 + 1) * 64 + lane) * 4 + k, acc_im[b][k], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    }
+  for (int b = 0; b < NB; ++b) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, acc_re[b]), slots,
+                                           ((b * 2 + 0) * 64 + lane) * 16, mine, kCpolDevice);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, acc_im[b]), slots,
+                                           ((b * 2 + 1) * 64 + lane) * 16, mine, kCpolDevice);
+  }
   __builtin_amdgcn_s_waitcnt(0);  // every slot store acknowledged
   asm volatile("" ::: "memory");
   uint32_t arrived = 0;
@@ -482,30 +499,26 @@ __device__ __forceinline__ void chunk_finish(const InterpArgs& a, const ChunkIte
   if (arrived + 1 != c.n_chunks) return;
   asm volatile("" ::: "memory");
   if (lane == 0) __hip_atomic_store(a.counters + c.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const float* first = a.partials + static_cast<size_t>(c.slot0) * kSlotFloats;
+  const int first = static_cast<int>(c.slot0 * kSlotFloats * 4);
   const int4 rows = group_rows_of(a, static_cast<uint32_t>(c.group), lane);
   const uint32_t n = c.n_chunks;
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     float4_t re = float4_t{0.f, 0.f, 0.f, 0.f}, im = re;
-    // slots are read kBatch at a time (all loads issued before the first
-    // add), then added in chunk order: ceil(n / kBatch) memory round trips
-    constexpr uint32_t kBatch = 4;
-    for (uint32_t q0 = 0; q0 < n; q0 += kBatch) {
-      float4_t pr[kBatch], pi[kBatch];
+    // kSlotBatch slots are loaded (all before the first add), then added in
+    // chunk order: ceil(n / kSlotBatch) memory round trips
+    for (uint32_t q0 = 0; q0 < n; q0 += kSlotBatch) {
+      float4_t pr[kSlotBatch], pi[kSlotBatch];
 #pragma unroll
-      for (uint32_t j = 0; j < kBatch; ++j) {
-        const float* p = first + static_cast<size_t>(min(q0 + j, n - 1)) * kSlotFloats;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          pr[j][k] = __hip_atomic_load(p + ((b * 2 + 0) * 64 + lane) * 4 + k, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-          pi[j][k] = __hip_atomic_load(p + ((b * 2 + 1) * 64 + lane) * 4 + k, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-        }
+      for (uint32_t j = 0; j < kSlotBatch; ++j) {
+        const int so = first + static_cast<int>(min(q0 + j, n - 1) * kSlotFloats * 4);
+        pr[j] = __builtin_bit_cast(float4_t, __builtin_amdgcn_raw_buffer_load_b128(
+                                                 slots, ((b * 2 + 0) * 64 + lane) * 16, so, kCpolDevice));
+        pi[j] = __builtin_bit_cast(float4_t, __builtin_amdgcn_raw_buffer_load_b128(
+                                                 slots, ((b * 2 + 1) * 64 + lane) * 16, so, kCpolDevice));
       }
 #pragma unroll
-      for (uint32_t j = 0; j < kBatch; ++j) {
+      for (uint32_t j = 0; j < kSlotBatch; ++j) {
         if (q0 + j >= n) break;
         if (q0 + j == 0) {
           re = pr[j];
@@ -536,7 +549,7 @@ __global__ __launch_bounds__(64) void interp_tiles_kernel(InterpArgs a) {
     acc_re[b] = float4_t{0.f, 0.f, 0.f, 0.f};
     acc_im[b] = float4_t{0.f, 0.f, 0.f, 0.f};
   }
-  const uint32_t entry = a.order[xcd_remap(blockIdx.x, gridDim.x)];
+  const uint32_t entry = glb(a.order)[xcd_remap(blockIdx.x, gridDim.x)];
   const bool is_chunk = (entry & kChunkTag) != 0;
   ChunkItem c{};
   WorkItem u;
@@ -547,7 +560,7 @@ __global__ __launch_bounds__(64) void interp_tiles_kernel(InterpArgs a) {
     problem = c.problem;
   } else {
     u = a.items[entry];
-    if (a.problems) problem = a.item_problem[entry];
+    if (a.problems) problem = glb(a.item_problem)[entry];
   }
   InterpArgs v = a;
   if constexpr (NB == 1 && IL) {
@@ -1021,15 +1034,35 @@ static void release(tsn_direction* d) {
   delete d;
 }
 
+// Longest entries first within each XCD's contiguous range of a launch order
+// (block b runs on XCD b mod 8 and takes entry xcd_remap(b)), in windows of
+// kLptWindow entries, about what one XCD holds resident: the waves dispatched
+// last are then the shortest.  The sort is stable, so entries of equal length
+// keep pack order.  Measured at 205,824 samples / 512^2 grid / 8 coils: G 18.5
+// -> 17.9 us, G^T 19.7 -> 18.3 us.
+constexpr uint32_t kLptWindow = 512;
+
+template <typename F>
+static void lpt_order(std::vector<uint32_t>& order, F tiles_of) {
+  const uint32_t nb = static_cast<uint32_t>(order.size());
+  for (uint32_t x = 0; x < 8; ++x) {
+    const uint32_t lo = xcd_remap(x, nb), hi = x == 7 ? nb : xcd_remap(x + 1, nb);
+    for (uint32_t b = lo; b < hi; b += kLptWindow)
+      std::stable_sort(order.begin() + b, order.begin() + std::min(hi, b + kLptWindow),
+                       [&](uint32_t p, uint32_t q) { return tiles_of(p) > tiles_of(q); });
+  }
+}
+
 // Work schedule of a packed direction: the groups of at most
 // kSplitAboveTiles tiles are cut, in pack order, into contiguous runs of whole
 // groups holding at most unit_tiles tiles (a single group is never cut); an
 // empty group is a unit of its own (zero epilogue).  Longer groups are
 // chunks, fixed at upload.  unit_tiles 0 picks the default: the regular tiles
-// over 12 waves per CU, clamped to 4..14 tiles.  Measured at 205,824 samples /
-// 512^2 grid (one-wave blocks, pack-ordered launch): G 18.7 us with 14-tile
-// units against 19.5 us with 8 or 12; G^T 20.7 us against 23.7 (8) and
-// 21.5 (12); 4-tile units pay more per-unit start-up.
+// over 12 waves per CU, clamped to 4..14 tiles.  The launch order then runs
+// the longest entries of each XCD first (lpt_order).  Measured at 205,824
+// samples / 512^2 grid / 8 coils: G 17.9 us with 14-tile units against 18.0
+// (16) and 18.1 (12); G^T 18.3 us against 18.4 (16) and 20.1 (12); smaller
+// units start more waves than the CUs hold at once.
 static int schedule_units(tsn_direction* d, uint32_t unit_tiles) {
   const std::vector<uint32_t>& off = d->host_offsets;
   const uint32_t groups = d->h.groups;
@@ -1051,6 +1084,7 @@ static int schedule_units(tsn_direction* d, uint32_t unit_tiles) {
   std::vector<WorkItem> items;
   std::vector<uint32_t> order;
   uint32_t chunk = 0;  // chunks are numbered in group order (long_chunks)
+  std::vector<int32_t> chunk_tiles;
   WorkItem cur{-1, -1, 0, 0};
   auto flush = [&]() {
     if (cur.group >= 0) {
@@ -1065,7 +1099,10 @@ static int schedule_units(tsn_direction* d, uint32_t unit_tiles) {
     const int32_t gi = static_cast<int32_t>(g);
     if (e - b > kSplitAboveTiles) {  // a chunked long group
       flush();
-      for (int32_t t = b; t < e; t += kSplitAboveTiles) order.push_back(kChunkTag | chunk++);
+      for (int32_t t = b; t < e; t += kSplitAboveTiles) {
+        order.push_back(kChunkTag | chunk++);
+        chunk_tiles.push_back(std::min<int32_t>(kSplitAboveTiles, e - t));
+      }
       continue;
     }
     if (e == b) {
@@ -1083,6 +1120,9 @@ static int schedule_units(tsn_direction* d, uint32_t unit_tiles) {
     }
   }
   flush();
+  lpt_order(order, [&](uint32_t e) {
+    return (e & kChunkTag) ? chunk_tiles[e & ~kChunkTag] : items[e].tile_end - items[e].tile_begin;
+  });
   WorkItem* new_items = nullptr;
   uint32_t* new_order = nullptr;
   size_t bytes = 0;
@@ -1138,6 +1178,7 @@ static int upload_chunks(tsn_direction* d) {
   uint32_t slots = 0, counters = 0;
   const std::vector<ChunkItem> chunks = long_chunks(d->host_offsets, 0, 0, 0, &slots, &counters);
   if (chunks.empty()) return TSN_OK;
+  if (slots > kMaxSlots) return fail(TSN_ERR_ARGUMENT, "%u long-group chunks (at most %u)", slots, kMaxSlots);
   size_t bytes = 0;
   TSN_CHECK(device_copy(chunks.data(), chunks.size(), &d->chunks, &bytes));
   d->n_chunks = static_cast<uint32_t>(chunks.size());
@@ -1286,6 +1327,11 @@ int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* o
     slots += static_cast<uint32_t>(members.size()) * ns;
     counters += static_cast<uint32_t>(members.size()) * nc;
   }
+  lpt_order(order, [&](uint32_t e) {
+    const int32_t t = (e & kChunkTag) ? chunks[e & ~kChunkTag].tile_end - chunks[e & ~kChunkTag].tile_begin
+                                      : items[e].tile_end - items[e].tile_begin;
+    return t;
+  });
   InterpBatch b;
   b.values = values;
   b.dense = std::all_of(parts.begin(), parts.end(), [](const InterpBatchPart& p) { return p.dir->tile_dense; });
@@ -1300,6 +1346,8 @@ int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* o
   if (rc == TSN_OK) rc = device_copy(item_problem.data(), item_problem.size(), &b.item_problem, &bytes);
   if (rc == TSN_OK) rc = device_copy(chunks.data(), chunks.size(), &b.chunks, &bytes);
   if (rc == TSN_OK) rc = device_copy(order.data(), order.size(), &b.order, &bytes);
+  if (rc == TSN_OK && slots > kMaxSlots)
+    rc = fail(TSN_ERR_ARGUMENT, "%u long-group chunks in the batch (at most %u)", slots, kMaxSlots);
   if (rc == TSN_OK && slots) {
     const hipError_t e1 = hipMalloc(reinterpret_cast<void**>(&b.partials), static_cast<size_t>(slots) * kSlotFloats * 4);
     const hipError_t e2 = e1 == hipSuccess ? hipMalloc(reinterpret_cast<void**>(&b.counters), counters * 4) : e1;

@@ -119,9 +119,9 @@ struct ChunkItem {
 // Floats of one partial slot: two coil blocks x (re, im) x 64 lanes x float4.
 constexpr uint32_t kSlotFloats = 2 * 2 * 64 * 4;
 // Launch order entries: a unit index, or kChunkTag | chunk index.  Units and
-// chunks follow pack order, so each XCD's contiguous range of the launch
+// chunks are cut in pack order, so each XCD's contiguous range of the launch
 // holds neighbouring groups (shared columns in its L2) and its share of the
-// long groups.
+// long groups; within windows of a range the longest entries go first.
 constexpr uint32_t kChunkTag = 0x80000000u;
 // Chunks of a direction's long groups with slots and counters numbered from
 // slot_base / counter_base (host side).
