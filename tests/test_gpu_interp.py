@@ -101,16 +101,23 @@ def test_fp32_csr_matches_oracle(view, coils):
     assert rel_l2(out.cpu().numpy(), ref) <= 1e-6
 
 
-def test_split_panel_is_bitwise_numpy_rounding():
+@pytest.mark.parametrize("coils,rows", [(12, 1000), (5, 130), (16, 64), (3, 1)])
+def test_split_panel_is_bitwise_numpy_rounding(coils, rows):
+    """Both split kernels (separate hi/residual panels and the interleaved
+    hi | res << 16 words, 64-row tiles) round like numpy, ragged row counts and
+    partial coil blocks included."""
+
     from trajsparsenufft_amd import device
     rng = np.random.default_rng(3)
-    coils, rows = 12, 1000
     v = (rng.standard_normal((coils, rows)) + 1j * rng.standard_normal((coils, rows))).astype(np.complex64)
-    v[0, :10] = [0, 1e-9, -1e-9, 65504, 1e-5, 3.14159, -0.0, 6e-8, 1e4, 2.5e-3]
+    special = [0, 1e-9, -1e-9, 65504, 1e-5, 3.14159, -0.0, 6e-8, 1e4, 2.5e-3]
+    v[0, :min(rows, 10)] = special[:min(rows, 10)]
     hi, res = device.split_panel(dev(v), coils, scale=0.25)
     hi, res = hi.cpu().numpy(), res.cpu().numpy()
+    il = device.split_panel_il(dev(v), coils, scale=0.25).cpu().numpy().view(np.uint32)
+    assert np.array_equal(il & 0xFFFF, hi.view(np.uint16)) and np.array_equal(il >> 16, res.view(np.uint16))
     s = v * np.float32(0.25)
-    for b in range(2):
+    for b in range((coils + 7) // 8):
         for j in range(8):
             c = 8 * b + j
             for part, ch in ((s.real, j), (s.imag, j + 8)):

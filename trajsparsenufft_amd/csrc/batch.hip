@@ -87,7 +87,7 @@ struct tsn_batch {
   uint32_t problems = 0, cmax = 0, bmax = 0, mmax = 0, pixels = 0, vec_blocks = 0, grid = 0;
   std::vector<uint32_t> coils, m;
   std::vector<const tsn_direction*> fwd, adj;
-  InterpBatch ib_fwd, ib_adj;
+  InterpBatch ib_fwd[2], ib_adj[2];  // parts of one and of two coil blocks
   std::vector<void*> allocations;
   size_t bytes = 0;
   float2 *tw0 = nullptr, *tw1 = nullptr, *smaps_t = nullptr, *t_fwd = nullptr, *t_inv = nullptr, *grid_buf = nullptr;
@@ -122,8 +122,10 @@ void batch_destroy(tsn_batch* bt) {
   if (!bt) return;
   if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
   if (bt->graph) (void)hipGraphDestroy(bt->graph);
-  interp_batch_free(&bt->ib_fwd);
-  interp_batch_free(&bt->ib_adj);
+  for (int k = 0; k < 2; ++k) {
+    interp_batch_free(&bt->ib_fwd[k]);
+    interp_batch_free(&bt->ib_adj[k]);
+  }
   for (void* q : bt->allocations) (void)hipFree(q);
   delete bt;
 }
@@ -184,7 +186,8 @@ int enqueue_batch_cg(tsn_batch* bt, uint32_t iterations, hipStream_t s) {
   else
     hipLaunchKernelGGL(batch_split_kernel<false>, split_grid, dim3(256), 0, s, bt->meas_tab, bt->coils_dev,
                        bt->m_dev, bt->bmax, norm, bt->smp_il, sample_stride(bt));
-  TSN_CHECK(launch_interp_batch(bt->ib_adj, kind, 1, true, 1.0f, s));
+  for (const InterpBatch& ib : bt->ib_adj)
+    if (ib.n_problems) TSN_CHECK(launch_interp_batch(ib, kind, 1, true, 1.0f, s));
   {
     FftArgs a = batch_fft_args(bt);
     a.image_out = bt->b;
@@ -220,8 +223,10 @@ int enqueue_batch_cg(tsn_batch* bt, uint32_t iterations, hipStream_t s) {
       std::swap(r_cur, r_other);
     }
     TSN_CHECK(fft_forward(a, out_kind_of(bt->cfg.precision), s, S));
-    TSN_CHECK(launch_interp_batch(bt->ib_fwd, kind, 3, false, norm, s));
-    TSN_CHECK(launch_interp_batch(bt->ib_adj, kind, 1, true, 1.0f, s));
+    for (const InterpBatch& ib : bt->ib_fwd)
+      if (ib.n_problems) TSN_CHECK(launch_interp_batch(ib, kind, 3, false, norm, s));
+    for (const InterpBatch& ib : bt->ib_adj)
+      if (ib.n_problems) TSN_CHECK(launch_interp_batch(ib, kind, 1, true, 1.0f, s));
     FftArgs inv = batch_fft_args(bt);
     inv.image_out = bt->ap;
     inv.reg_in = p_cur;
@@ -387,21 +392,23 @@ int tsn_batch_create(const tsn_plan_config* cfg, const tsn_batch_problem* proble
     const hipError_t e = hipDeviceSynchronize();
     if (e != hipSuccess) rc = fail(TSN_ERR_RUNTIME, "batch setup: %s", hipGetErrorString(e));
   }
-  // interpolation parts: one per coil block of each problem
+  // interpolation parts: one per problem, in the batch of its coil-block
+  // count (block 1's panels follow block 0's in pan_il and smp_il)
   if (rc == TSN_OK) {
-    std::vector<InterpBatchPart> fw, ad;
+    std::vector<InterpBatchPart> fw[2], ad[2];
     for (uint32_t z = 0; z < count; ++z) {
       const uint32_t blocks = div_up(bt->coils[z], kCoilsPerBlock);
-      for (uint32_t k = 0; k < blocks; ++k) {
-        uint32_t* pan = bt->pan_il + z * panel_stride(bt) + static_cast<uint64_t>(k) * grid * kN;
-        uint32_t* smp = bt->smp_il + z * sample_stride(bt) + static_cast<uint64_t>(k) * bt->m[z] * kN;
-        float* grid_out = reinterpret_cast<float*>(bt->grid_buf + z * grid * bt->cmax);
-        fw.push_back({bt->fwd[z], pan, nullptr, smp, bt->cmax, k * kCoilsPerBlock});
-        ad.push_back({bt->adj[z], smp, grid_out, nullptr, bt->cmax, k * kCoilsPerBlock});
-      }
+      uint32_t* pan = bt->pan_il + z * panel_stride(bt);
+      uint32_t* smp = bt->smp_il + z * sample_stride(bt);
+      float* grid_out = reinterpret_cast<float*>(bt->grid_buf + z * grid * bt->cmax);
+      fw[blocks - 1].push_back({bt->fwd[z], pan, nullptr, smp, bt->cmax, 0, blocks});
+      ad[blocks - 1].push_back({bt->adj[z], smp, grid_out, nullptr, bt->cmax, 0, blocks});
     }
-    rc = interp_batch_build(fw, &bt->ib_fwd);
-    if (rc == TSN_OK) rc = interp_batch_build(ad, &bt->ib_adj);
+    for (int k = 0; k < 2 && rc == TSN_OK; ++k) {
+      if (fw[k].empty()) continue;
+      rc = interp_batch_build(fw[k], &bt->ib_fwd[k]);
+      if (rc == TSN_OK) rc = interp_batch_build(ad[k], &bt->ib_adj[k]);
+    }
   }
   if (rc != TSN_OK) {
     batch_destroy(bt);

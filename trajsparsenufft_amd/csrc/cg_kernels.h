@@ -98,6 +98,57 @@ __global__ __launch_bounds__(kVecThreads) void cg_final_kernel(
   for (uint32_t i = threadIdx.x; i <= iter + 1; i += blockDim.x) rho_hist[i] = sqrt(fmax(rho_hist[i], 0.0));
 }
 
+// The last CG step fused with the result (single plans): alpha as in
+// cg_final_kernel, then x_out = transpose(x + alpha p), [cols][rows] from the
+// column-major [rows][cols] vectors, in 32 x 32 tiles through LDS (256
+// threads: 32 columns x 8 rows).  Block (0, 0) also writes rho_hist[iter + 1],
+// turns the history into sqrt(rho) and copies the status word to status_out.
+__global__ __launch_bounds__(kVecThreads) void cg_final_out_kernel(
+    const float2* __restrict__ x, const float2* __restrict__ p, const double* __restrict__ dots,
+    uint32_t dot_count, double* __restrict__ rho_hist, uint32_t iter, int* __restrict__ status,
+    int* __restrict__ status_out, uint32_t rows, uint32_t cols, float2* __restrict__ x_out) {
+  __shared__ double red[kVecThreads];
+  __shared__ float2 tile[32][33];
+  const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const uint32_t bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  // issue the tile loads before the reductions so their latency overlaps
+  float2 xv[4], pv[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t r = by + ty + 8 * k, c = bx + tx;
+    const bool in = r < rows && c < cols;
+    xv[k] = in ? x[static_cast<size_t>(r) * cols + c] : make_float2(0.f, 0.f);
+    pv[k] = in ? p[static_cast<size_t>(r) * cols + c] : make_float2(0.f, 0.f);
+  }
+  const double pap = sum_partials(dots, dot_count, red);
+  __syncthreads();
+  const double rap = sum_partials(dots + dot_count, dot_count, red);
+  __syncthreads();
+  const double apap = sum_partials(dots + 2 * dot_count, dot_count, red);
+  const double rho = rho_hist[iter];
+  const bool first = blockIdx.x == 0 && blockIdx.y == 0;
+  double alpha = 0.0;
+  if (isfinite(pap) && pap > 0.0) alpha = rho / pap;
+  else if (first && threadIdx.x == 0) *status = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    tile[ty + 8 * k][tx] = make_float2(static_cast<float>(xv[k].x + alpha * pv[k].x),
+                                       static_cast<float>(xv[k].y + alpha * pv[k].y));
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t c = bx + ty + 8 * k, r = by + tx;
+    if (r < rows && c < cols) x_out[static_cast<size_t>(c) * rows + r] = tile[tx][ty + 8 * k];
+  }
+  if (!first) return;
+  if (threadIdx.x == 0) {
+    rho_hist[iter + 1] = fmax(rho - 2.0 * alpha * rap + alpha * alpha * apap, 0.0);
+    *status_out = *status;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i <= iter + 1; i += blockDim.x) rho_hist[i] = sqrt(fmax(rho_hist[i], 0.0));
+}
+
 // rho_hist[index] = sum of partials (status = 1 when index == 0).  With
 // finish, rho_hist[0 .. index] then become the residual history sqrt(rho).
 __global__ __launch_bounds__(kVecThreads) void cg_rho_kernel(const double* __restrict__ partial,
@@ -175,10 +226,6 @@ __global__ __launch_bounds__(256) void samples_to_rows_kernel(const float2* __re
   const uint32_t c = static_cast<uint32_t>(e % coils), row = static_cast<uint32_t>(e / coils);
   const float2 v = in[static_cast<size_t>(c) * m + row];
   out[e] = make_float2(v.x * scale, v.y * scale);
-}
-
-__global__ void copy_status_kernel(const int* __restrict__ src, int* __restrict__ dst) {
-  if (threadIdx.x == 0) *dst = *src;
 }
 
 // a fill kernel (not hipMemsetAsync) so the work can sit inside a captured graph

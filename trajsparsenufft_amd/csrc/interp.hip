@@ -114,7 +114,7 @@ __device__ __forceinline__ InterpArgs problem_args(const InterpArgs& a, uint32_t
   v.coils = q.coils;
   v.coil_base = q.coil_base;
   v.panel_il = q.panel_il;
-  v.panel_stride = static_cast<uint64_t>(q.n_cols) * kN;  // one block: its rows bound the buffer range
+  v.panel_stride = static_cast<uint64_t>(q.n_cols) * kN;  // block b's input panel follows at b strides
   v.out_f32 = q.out_f32;
   v.out_il = q.out_il;
   return v;
@@ -563,7 +563,7 @@ __global__ __launch_bounds__(64) void interp_tiles_kernel(InterpArgs a) {
     if (a.problems) problem = glb(a.item_problem)[entry];
   }
   InterpArgs v = a;
-  if constexpr (NB == 1 && IL) {
+  if constexpr (IL) {
     if (a.problems) v = problem_args(a, problem);
   }
   if (u.tile_begin == u.tile_end) {  // empty group: zero rows
@@ -713,25 +713,34 @@ __global__ __launch_bounds__(256) void split_panel_kernel(const float2* __restri
 }
 
 // Interleaved variant for the plan: one uint32 = hi | res << 16 per channel,
-// FP16 (RN) or, with BF16, bfloat16 (RNE) halves.
+// FP16 (RN) or, with BF16, bfloat16 (RNE) halves.  A block converts 64 rows of
+// one coil block: each wave reads 64 consecutive rows of one coil (512
+// contiguous bytes), the tile is turned through LDS, and the panel rows go
+// out as 64 contiguous bytes per row.
+constexpr uint32_t kSplitRows = 64;
+
 template <bool BF16>
 __global__ __launch_bounds__(256) void split_panel_il_kernel(const float2* __restrict__ in, uint32_t rows,
-                                                             uint32_t coils, uint32_t blocks, float scale,
+                                                             uint32_t coils, float scale,
                                                              uint32_t* __restrict__ out) {
-  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const uint64_t total = static_cast<uint64_t>(blocks) * rows * kN;
-  if (gtid >= total) return;
-  const int n = static_cast<int>(gtid & 15);
-  const uint64_t br = gtid >> 4;
-  const uint32_t row = static_cast<uint32_t>(br % rows);
-  const uint32_t b = static_cast<uint32_t>(br / rows);
-  const uint32_t coil = b * kCoilsPerBlock + (n & 7);
-  float v = 0.f;
-  if (coil < coils) {
-    const float2 x = in[static_cast<size_t>(coil) * rows + row];
-    v = (n < 8 ? x.x : x.y) * scale;
+  __shared__ float2 tile[kSplitRows][kCoilsPerBlock + 1];
+  const uint32_t row0 = blockIdx.x * kSplitRows, b = blockIdx.y;
+#pragma unroll
+  for (uint32_t k = 0; k < kSplitRows * kCoilsPerBlock / 256; ++k) {
+    const uint32_t i = threadIdx.x + 256 * k;
+    const uint32_t c = i / kSplitRows, r = i % kSplitRows;
+    const uint32_t coil = b * kCoilsPerBlock + c, row = row0 + r;
+    tile[r][c] = (coil < coils && row < rows) ? in[static_cast<size_t>(coil) * rows + row] : make_float2(0.f, 0.f);
   }
-  out[gtid] = split_word<BF16>(v);
+  __syncthreads();
+#pragma unroll
+  for (uint32_t k = 0; k < kSplitRows * kN / 256; ++k) {
+    const uint32_t w = threadIdx.x + 256 * k;
+    const uint32_t r = w / kN, n = w % kN, row = row0 + r;
+    if (row >= rows) continue;
+    const float2 x = tile[r][n & 7];
+    out[(static_cast<size_t>(b) * rows + row) * kN + n] = split_word<BF16>((n < 8 ? x.x : x.y) * scale);
+  }
 }
 
 // Materialized data consistency: out = split((pred - scale * y) * w) with
@@ -1000,13 +1009,11 @@ int launch_split_panel_il(const float2* in, uint32_t rows, uint32_t coils, float
   if (values != TSN_VALUES_FP16 && values != TSN_VALUES_BF16)
     return fail(TSN_ERR_ARGUMENT, "unknown value format %d", values);
   const uint32_t blocks = div_up(coils, kCoilsPerBlock);
-  const uint64_t total = static_cast<uint64_t>(blocks) * rows * kN;
+  const dim3 grid(div_up(rows, kSplitRows), blocks);
   if (values == TSN_VALUES_BF16)
-    hipLaunchKernelGGL(split_panel_il_kernel<true>, dim3(div_up(total, 256)), dim3(256), 0, stream, in, rows,
-                       coils, blocks, scale, out);
+    hipLaunchKernelGGL(split_panel_il_kernel<true>, grid, dim3(256), 0, stream, in, rows, coils, scale, out);
   else
-    hipLaunchKernelGGL(split_panel_il_kernel<false>, dim3(div_up(total, 256)), dim3(256), 0, stream, in, rows,
-                       coils, blocks, scale, out);
+    hipLaunchKernelGGL(split_panel_il_kernel<false>, grid, dim3(256), 0, stream, in, rows, coils, scale, out);
   TSN_HIP(hipGetLastError());
   return TSN_OK;
 }
@@ -1272,11 +1279,14 @@ int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* o
   *out = InterpBatch{};
   if (parts.empty()) return fail(TSN_ERR_ARGUMENT, "empty interpolation batch");
   const int values = parts[0].dir->values;
+  const uint32_t blocks = parts[0].blocks;
+  if (blocks != 1 && blocks != 2) return fail(TSN_ERR_ARGUMENT, "a batch part covers 1 or 2 coil blocks");
   std::vector<InterpProblem> probs;
   for (const InterpBatchPart& p : parts) {
     const tsn_direction* d = p.dir;
     if (!d || d->csr) return fail(TSN_ERR_ARGUMENT, "batched interpolation needs packed directions");
     if (d->values != values) return fail(TSN_ERR_ARGUMENT, "a batch mixes tile value formats");
+    if (p.blocks != blocks) return fail(TSN_ERR_ARGUMENT, "a batch mixes parts of 1 and 2 coil blocks");
     InterpProblem q{};
     q.group_rows = d->group_rows;
     q.tile_cols = d->tile_cols;
@@ -1294,13 +1304,20 @@ int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* o
     q.out_il = p.out_il;
     probs.push_back(q);
   }
-  // parts grouped by direction, in order of first appearance; unit-major within a group
+  // Parts are grouped by direction, in order of first appearance.  Each
+  // direction's own launch order (cut into one range per XCD, longest first)
+  // is split at its XCD boundaries, and XCD x of the batch runs piece x of
+  // every direction, one after the other: as in a launch of its own, each
+  // XCD's L2 holds one eighth of a panel's columns at a time.  Within a
+  // piece the members of a direction (coil blocks, frames) follow each
+  // entry, so they reuse its tile operands from L2.
   std::vector<const tsn_direction*> dirs;
   for (const InterpBatchPart& p : parts)
     if (std::find(dirs.begin(), dirs.end(), p.dir) == dirs.end()) dirs.push_back(p.dir);
   std::vector<WorkItem> items;
-  std::vector<uint32_t> item_problem, order;
+  std::vector<uint32_t> item_problem;
   std::vector<ChunkItem> chunks;
+  std::vector<uint32_t> xcd_entries[8];
   uint32_t slots = 0, counters = 0;
   for (const tsn_direction* d : dirs) {
     std::vector<uint32_t> members;
@@ -1309,30 +1326,36 @@ int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* o
     // every part has its own slots and counters
     uint32_t ns = 0, nc = 0;
     const std::vector<ChunkItem> dc = long_chunks(d->host_offsets, 0, 0, 0, &ns, &nc);
-    for (uint32_t e : d->host_order)
-      for (uint32_t k = 0; k < members.size(); ++k) {
-        if (e & kChunkTag) {
-          ChunkItem c = dc[e & ~kChunkTag];
-          c.slot0 += slots + k * ns;
-          c.counter += counters + k * nc;
-          c.problem = members[k];
-          order.push_back(kChunkTag | static_cast<uint32_t>(chunks.size()));
-          chunks.push_back(c);
-        } else {
-          order.push_back(static_cast<uint32_t>(items.size()));
-          items.push_back(d->host_items[e]);
-          item_problem.push_back(members[k]);
+    const uint32_t nd = static_cast<uint32_t>(d->host_order.size());
+    for (uint32_t x = 0; x < 8; ++x) {
+      const uint32_t lo = xcd_remap(x, nd), hi = x == 7 ? nd : xcd_remap(x + 1, nd);
+      for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t e = d->host_order[i];
+        for (uint32_t k = 0; k < members.size(); ++k) {
+          if (e & kChunkTag) {
+            ChunkItem c = dc[e & ~kChunkTag];
+            c.slot0 += slots + k * ns;
+            c.counter += counters + k * nc;
+            c.problem = members[k];
+            xcd_entries[x].push_back(kChunkTag | static_cast<uint32_t>(chunks.size()));
+            chunks.push_back(c);
+          } else {
+            xcd_entries[x].push_back(static_cast<uint32_t>(items.size()));
+            items.push_back(d->host_items[e]);
+            item_problem.push_back(members[k]);
+          }
         }
       }
+    }
     slots += static_cast<uint32_t>(members.size()) * ns;
     counters += static_cast<uint32_t>(members.size()) * nc;
   }
-  lpt_order(order, [&](uint32_t e) {
-    const int32_t t = (e & kChunkTag) ? chunks[e & ~kChunkTag].tile_end - chunks[e & ~kChunkTag].tile_begin
-                                      : items[e].tile_end - items[e].tile_begin;
-    return t;
-  });
+  // The launch takes XCD ranges of equal length (xcd_remap), so a few entries
+  // at each boundary run on the neighbouring XCD.
+  std::vector<uint32_t> order;
+  for (const std::vector<uint32_t>& v : xcd_entries) order.insert(order.end(), v.begin(), v.end());
   InterpBatch b;
+  b.blocks = blocks;
   b.values = values;
   b.dense = std::all_of(parts.begin(), parts.end(), [](const InterpBatchPart& p) { return p.dir->tile_dense; });
   b.n_problems = static_cast<uint32_t>(probs.size());
@@ -1370,11 +1393,17 @@ void interp_batch_free(InterpBatch* b) {
   *b = InterpBatch{};
 }
 
+template <int KIND, int NB>
+static void launch_batch_nb(const InterpArgs& a, int out_mode, bool conj, hipStream_t s) {
+  if (out_mode == kOutRowMajor) launch_tiles_om<KIND, NB, true, kOutRowMajor>(a, conj, false, s);
+  else if (KIND == kSparseSplitBF16) launch_tiles_om<KIND, NB, true, kOutSplitBF16IL>(a, conj, false, s);
+  else launch_tiles_om<KIND, NB, true, kOutSplit16IL>(a, conj, false, s);
+}
+
 template <int KIND>
-static void launch_batch_kind(const InterpArgs& a, int out_mode, bool conj, hipStream_t s) {
-  if (out_mode == kOutRowMajor) launch_tiles_om<KIND, 1, true, kOutRowMajor>(a, conj, false, s);
-  else if (KIND == kSparseSplitBF16) launch_tiles_om<KIND, 1, true, kOutSplitBF16IL>(a, conj, false, s);
-  else launch_tiles_om<KIND, 1, true, kOutSplit16IL>(a, conj, false, s);
+static void launch_batch_kind(const InterpArgs& a, uint32_t blocks, int out_mode, bool conj, hipStream_t s) {
+  if (blocks == 2) launch_batch_nb<KIND, 2>(a, out_mode, conj, s);
+  else launch_batch_nb<KIND, 1>(a, out_mode, conj, s);
 }
 
 int launch_interp_batch(const InterpBatch& b, int kind, int out_mode, bool conj, float scale, hipStream_t s) {
@@ -1397,10 +1426,10 @@ int launch_interp_batch(const InterpBatch& b, int kind, int out_mode, bool conj,
   a.item_problem = b.item_problem;
   a.coils = kCoilsPerBlock;  // replaced per problem
   switch (kind) {
-    case kSparseSplit: launch_batch_kind<kSparseSplit>(a, out_mode, conj, s); break;
-    case kDenseSplit: launch_batch_kind<kDenseSplit>(a, out_mode, conj, s); break;
-    case kSparseHi: launch_batch_kind<kSparseHi>(a, out_mode, conj, s); break;
-    case kSparseSplitBF16: launch_batch_kind<kSparseSplitBF16>(a, out_mode, conj, s); break;
+    case kSparseSplit: launch_batch_kind<kSparseSplit>(a, b.blocks, out_mode, conj, s); break;
+    case kDenseSplit: launch_batch_kind<kDenseSplit>(a, b.blocks, out_mode, conj, s); break;
+    case kSparseHi: launch_batch_kind<kSparseHi>(a, b.blocks, out_mode, conj, s); break;
+    case kSparseSplitBF16: launch_batch_kind<kSparseSplitBF16>(a, b.blocks, out_mode, conj, s); break;
     default: return fail(TSN_ERR_ARGUMENT, "unknown interpolation kind %d", kind);
   }
   TSN_HIP(hipGetLastError());

@@ -231,10 +231,10 @@ int enqueue_cg(tsn_plan* pl, const float2* measurement, uint32_t iterations, flo
     TSN_CHECK(op_adjoint_grid(pl, s));
     TSN_CHECK(op_adjoint_image(pl, pl->ap, p_cur, scaled_lambda, inv_scale_sq, pl->partial, s, r_cur));
   }
-  hipLaunchKernelGGL(cg_final_kernel, dim3(pl->vec_blocks), dim3(kVecThreads), 0, s, pl->x, p_cur, pl->partial,
-                     pl->cfg.image1, history, iterations - 1, pl->status, n, uint64_t{0}, 0u, 0u);
-  TSN_CHECK(transpose_image(pl->x, pl->cfg.image1, pl->cfg.image0, 1.0f, x_out, s));
-  hipLaunchKernelGGL(copy_status_kernel, dim3(1), dim3(64), 0, s, pl->status, status_out);
+  // the last step, x_out = transpose(x) and the status copy in one kernel
+  hipLaunchKernelGGL(cg_final_out_kernel, dim3(div_up(pl->cfg.image0, 32), div_up(pl->cfg.image1, 32)),
+                     dim3(kVecThreads), 0, s, pl->x, p_cur, pl->partial, pl->cfg.image1, history, iterations - 1,
+                     pl->status, status_out, pl->cfg.image1, pl->cfg.image0, x_out);
   TSN_HIP(hipGetLastError());
   return TSN_OK;
 }

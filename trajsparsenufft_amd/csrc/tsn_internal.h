@@ -45,8 +45,6 @@ int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
 
-// bfloat16 bits of v, round to nearest even: the top half of the float32 bits
-// after adding 0x7FFF plus the lowest kept bit; NaN stays a quiet NaN.
 // Block b of a launch runs on XCD b % 8 (the dispatcher's round robin); this
 // gives each XCD a contiguous range of the nb work indices (a permutation of
 // 0 .. nb-1), so neighbouring work shares that XCD's L2.
@@ -56,6 +54,8 @@ __host__ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nb) {
   return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
+// bfloat16 bits of v, round to nearest even: the top half of the float32 bits
+// after adding 0x7FFF plus the lowest kept bit; NaN stays a quiet NaN.
 __host__ __device__ inline uint16_t bf16_rne_bits(float v) {
   uint32_t u = __builtin_bit_cast(uint32_t, v);
   if ((u & 0x7FFFFFFFu) > 0x7F800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);
@@ -127,19 +127,21 @@ constexpr uint32_t kChunkTag = 0x80000000u;
 // slot_base / counter_base (host side).
 std::vector<ChunkItem> long_chunks(const std::vector<uint32_t>& group_offsets, uint32_t problem, uint32_t slot_base,
                                    uint32_t counter_base, uint32_t* slots, uint32_t* counters);
-// One coil block of one problem of a batched interpolation launch.
+// One part of a batched interpolation launch: one or two coil blocks of one
+// problem (with two, one wave computes both from the same tile operands).
 struct InterpBatchPart {
   const tsn_direction* dir;
-  const uint32_t* panel_il;  // the block's interleaved input panel [n_cols][16]
+  const uint32_t* panel_il;  // the first block's interleaved input panel [n_cols][16]
   float* out_f32;            // the problem's float2 output (row- or coil-major)
-  uint32_t* out_il;          // the block's interleaved split output panel [n_out][16]
+  uint32_t* out_il;          // the first block's interleaved split output panel [n_out][16]
   uint32_t coils;            // coils of the problem
-  uint32_t coil_base;        // first coil of the block
+  uint32_t coil_base;        // first coil of the first block
+  uint32_t blocks = 1;       // coil blocks of the part (1 or 2; block 1's panels follow block 0's)
 };
 
-// Device tables of a batched launch: parts sharing a direction are merged
-// unit-major (consecutive waves of a block take the same unit of different
-// parts, so its tiles are read from HBM once and then from cache).
+// Device tables of a batched launch: XCD x runs piece x of every direction's
+// own launch order, and the parts sharing a direction follow each entry (so
+// its tiles are read from HBM once and then from cache).
 struct InterpBatch {
   void* problems = nullptr;
   WorkItem* items = nullptr;
@@ -149,6 +151,7 @@ struct InterpBatch {
   uint32_t* counters = nullptr;  // one per problem x long group, zero between launches
   uint32_t* order = nullptr;     // launch order of units and chunks (kChunkTag marks a chunk)
   uint32_t n_problems = 0, n_items = 0, n_chunks = 0;
+  uint32_t blocks = 1;  // coil blocks per part (every part of a batch has the same)
   int values = TSN_VALUES_FP16;
   bool dense = false;
 };
