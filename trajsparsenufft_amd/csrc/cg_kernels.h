@@ -74,11 +74,8 @@ __global__ __launch_bounds__(kVecThreads) void cg_final_kernel(
   dots += z * dstride;
   rho_hist += z * hstride;
   status += z;
-  const double pap = sum_partials(dots, dot_count, red);
-  __syncthreads();
-  const double rap = sum_partials(dots + dot_count, dot_count, red);
-  __syncthreads();
-  const double apap = sum_partials(dots + 2 * dot_count, dot_count, red);
+  double pap, rap, apap;
+  tsn::sum_partials3<kVecThreads / 64>(dots, dot_count, pap, rap, apap, red);
   const double rho = rho_hist[iter];
   double alpha = 0.0;
   if (isfinite(pap) && pap > 0.0) alpha = rho / pap;
@@ -120,11 +117,8 @@ __global__ __launch_bounds__(kVecThreads) void cg_final_out_kernel(
     xv[k] = in ? x[static_cast<size_t>(r) * cols + c] : make_float2(0.f, 0.f);
     pv[k] = in ? p[static_cast<size_t>(r) * cols + c] : make_float2(0.f, 0.f);
   }
-  const double pap = sum_partials(dots, dot_count, red);
-  __syncthreads();
-  const double rap = sum_partials(dots + dot_count, dot_count, red);
-  __syncthreads();
-  const double apap = sum_partials(dots + 2 * dot_count, dot_count, red);
+  double pap, rap, apap;
+  tsn::sum_partials3<kVecThreads / 64>(dots, dot_count, pap, rap, apap, red);
   const double rho = rho_hist[iter];
   const bool first = blockIdx.x == 0 && blockIdx.y == 0;
   double alpha = 0.0;

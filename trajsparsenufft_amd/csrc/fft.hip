@@ -29,7 +29,7 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
   const FftArgs a = problem_view(args, blockIdx.z);
   if (blockIdx.y * 8 >= a.coils) return;  // coil chunk beyond this problem (batched launches)
   extern __shared__ float2 lds[];
-  __shared__ double red[512];
+  __shared__ double red[24];
   constexpr int K = 64 * R;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   // neighbouring columns on one XCD: their 64-byte rows of T fill whole L2 lines
@@ -63,9 +63,8 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
   }
   if (a.cg_r) {
     // the previous step's scalars: every block reduces the same partials in the same order
-    const double pap = sum_partials512(a.cg_dots, a.dot_count, red);
-    const double rap = sum_partials512(a.cg_dots + a.dot_count, a.dot_count, red);
-    const double apap = sum_partials512(a.cg_dots + 2 * a.dot_count, a.dot_count, red);
+    double pap, rap, apap;
+    sum_partials3<8>(a.cg_dots, a.dot_count, pap, rap, apap, red);
     const double rho_old = a.rho_hist[a.iter - 1];
     const bool first_block = blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0;
     double alpha = 0.0;
@@ -200,7 +199,7 @@ __global__ __launch_bounds__(512) void inv_axis0_kernel(FftArgs args) {
   const FftArgs a = problem_view(args, blockIdx.z);
   const WaveTwiddles<R> twp = load_twiddles<R>(a.tw0, static_cast<int>(threadIdx.x & 63));
   extern __shared__ float2 lds[];
-  __shared__ double red[512];
+  __shared__ double red[24];
   constexpr int K = 64 * R;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t n1 = blockIdx.x;
@@ -263,18 +262,15 @@ __global__ __launch_bounds__(512) void inv_axis0_kernel(FftArgs args) {
     a.image_out[pix] = v;
   }
   if (a.dot_partial) {
-    const double d = block_sum512(part, red);
-    if (threadIdx.x == 0) a.dot_partial[n1] = d;
-    if (a.dot_r) {
-      const double dr = block_sum512(part_r, red);
-      const double dv = block_sum512(part_v, red);
-      if (threadIdx.x == 0) {
-        a.dot_partial[a.n1 + n1] = dr;
-        a.dot_partial[2 * a.n1 + n1] = dv;
+    block_sum3<8>(part, part_r, part_v, red);
+    if (threadIdx.x == 0) {
+      a.dot_partial[n1] = part;
+      if (a.dot_r) {
+        a.dot_partial[a.n1 + n1] = part_r;
+        a.dot_partial[2 * a.n1 + n1] = part_v;
       }
     }
-  }
-}
+  }}
 
 // ------------------------------------------------------------- launchers
 

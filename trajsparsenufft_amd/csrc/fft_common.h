@@ -65,25 +65,52 @@ struct FftArgs {
   uint64_t hist_stride;      // rho_hist (cg_status: one int per problem)
 };
 
-// Sum of v over a 512-thread block through red[512], in a fixed order; every
-// thread gets the sum.  Ends with a barrier, so red can be reused at once.
-__device__ __forceinline__ double block_sum512(double v, double* red) {
-  red[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = 256; off > 0; off >>= 1) {
-    if (static_cast<int>(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
-    __syncthreads();
+// Sums of three values over a block of WAVES waves, in a fixed order, every
+// thread getting all three: a butterfly over each wave (the two operands of
+// every add are the same in all lanes, so all lanes hold the same bits), then
+// the wave sums added in wave order.  One barrier pair; red holds 3 * WAVES
+// doubles.  Every block running it on the same inputs gets the same bits.
+template <int WAVES>
+__device__ __forceinline__ void block_sum3(double& a, double& b, double& c, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+    c += __shfl_xor(c, off);
   }
-  const double s = red[0];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[w] = a;
+    red[WAVES + w] = b;
+    red[2 * WAVES + w] = c;
+  }
   __syncthreads();
-  return s;
+  a = red[0];
+  b = red[WAVES];
+  c = red[2 * WAVES];
+#pragma unroll
+  for (int i = 1; i < WAVES; ++i) {
+    a += red[i];
+    b += red[WAVES + i];
+    c += red[2 * WAVES + i];
+  }
+  __syncthreads();
 }
 
-// Fixed-order sum of count partials (512-thread block).
-__device__ __forceinline__ double sum_partials512(const double* __restrict__ p, uint32_t count, double* red) {
-  double s = 0.0;
-  for (uint32_t i = threadIdx.x; i < count; i += 512) s += p[i];
-  return block_sum512(s, red);
+// Fixed-order sums of the three partial arrays p[0 .. count), p[count ..
+// 2 count), p[2 count .. 3 count) over a block of WAVES waves (1-D), every
+// load issued before the first add.
+template <int WAVES>
+__device__ __forceinline__ void sum_partials3(const double* __restrict__ p, uint32_t count, double& a, double& b,
+                                              double& c, double* red) {
+  a = b = c = 0.0;
+  for (uint32_t i = threadIdx.x; i < count; i += 64 * WAVES) {
+    const double x = p[i], y = p[count + i], z = p[2 * count + i];
+    a += x;
+    b += y;
+    c += z;
+  }
+  block_sum3<WAVES>(a, b, c, red);
 }
 
 // The arguments of problem z of a batched launch.
