@@ -37,35 +37,61 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
   const uint32_t c = chunk * 8 + w;
   const bool active = c < a.coils;
   const WaveTwiddles<R> twp = load_twiddles<R>(a.tw0, lane);
-  // issue every data load first (image, or r, p, Ap and x; maps,
-  // apodization); the reductions of the CG scalars then overlap their latency
+  // issue every load first: the previous step's partials and rho (needed
+  // first), then the image, or r, p, Ap and x, maps and apodization; the
+  // reduction of the CG scalars then overlaps the latency of the later loads
+  double q0 = 0.0, q1 = 0.0, q2 = 0.0, rho_old = 0.0;
+  if (a.cg_r) {  // (threads past dot_count load its last entry and drop it below)
+    rho_old = a.rho_hist[a.iter - 1];
+    const uint32_t qi = min(threadIdx.x, a.dot_count - 1);
+    q0 = a.cg_dots[qi];
+    q1 = a.cg_dots[a.dot_count + qi];
+    q2 = a.cg_dots[2 * a.dot_count + qi];
+  }
   const bool store_vec = chunk == 0 && w == 0;  // the wave that writes x, r and p_new
+  // Loads are unconditional, at clamped indices (straight-line code lets every
+  // load issue before the first wait; a load under a lane-dependent branch
+  // would be waited for at the branch's end); out-of-range values are zeroed
+  // afterwards.  Only j < JN can hold pixels: N0 = K/2.
+  constexpr int JN = R >= 2 ? R / 2 : 1;
+  const float2* src_r = a.cg_r ? a.cg_r : a.image_t;
+  const float2* src_p = a.cg_r ? a.cg_p_old : a.image_t;
+  const float2* src_a = a.cg_r ? a.cg_ap : a.image_t;
+  const float2* src_s = a.smaps_t + (static_cast<size_t>(n1) * a.cstride + min(c, a.coils - 1)) * a.n0;
+  const bool x_wave = a.cg_r && __builtin_amdgcn_readfirstlane(store_vec ? 1 : 0);
   float2 img[R], sm[R], pv[R], av[R], xv[R];
   float ap[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     img[j] = pv[j] = sm[j] = av[j] = xv[j] = make_float2(0.f, 0.f);
     ap[j] = 0.f;
-    const uint32_t n0 = lane + 64 * j;
-    if (active && n0 < a.n0) {
-      const size_t pix = static_cast<size_t>(n1) * a.n0 + n0;
-      if (a.cg_r) {
-        img[j] = a.cg_r[pix];
-        pv[j] = a.cg_p_old[pix];
-        av[j] = a.cg_ap[pix];
-        if (store_vec) xv[j] = a.cg_x[pix];
-      } else {
-        img[j] = a.image_t[pix];
-      }
-      sm[j] = a.smaps_t[(static_cast<size_t>(n1) * a.cstride + c) * a.n0 + n0];
-      ap[j] = a.apod_t[pix];
+    if (j >= JN) continue;
+    const uint32_t n0 = min(lane + 64u * j, a.n0 - 1);
+    const size_t pix = static_cast<size_t>(n1) * a.n0 + n0;
+    img[j] = src_r[pix];
+    pv[j] = src_p[pix];
+    av[j] = src_a[pix];
+    sm[j] = src_s[n0];
+    ap[j] = a.apod_t[pix];
+    if (x_wave) xv[j] = a.cg_x[pix];
+  }
+#pragma unroll
+  for (int j = 0; j < JN; ++j) {
+    if (!active || lane + 64u * j >= a.n0) {
+      img[j] = pv[j] = sm[j] = av[j] = xv[j] = make_float2(0.f, 0.f);
+      ap[j] = 0.f;
     }
   }
   if (a.cg_r) {
     // the previous step's scalars: every block reduces the same partials in the same order
-    double pap, rap, apap;
-    sum_partials3<8>(a.cg_dots, a.dot_count, pap, rap, apap, red);
-    const double rho_old = a.rho_hist[a.iter - 1];
+    if (threadIdx.x >= a.dot_count) q0 = q1 = q2 = 0.0;
+    for (uint32_t i = threadIdx.x + 512; i < a.dot_count; i += 512) {
+      q0 += a.cg_dots[i];
+      q1 += a.cg_dots[a.dot_count + i];
+      q2 += a.cg_dots[2 * a.dot_count + i];
+    }
+    block_sum3<8>(q0, q1, q2, red);
+    const double pap = q0, rap = q1, apap = q2;
     const bool first_block = blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0;
     double alpha = 0.0;
     if (isfinite(pap) && pap > 0.0) alpha = rho_old / pap;
@@ -203,32 +229,47 @@ __global__ __launch_bounds__(512) void inv_axis0_kernel(FftArgs args) {
   constexpr int K = 64 * R;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t n1 = blockIdx.x;
+  constexpr int TPL = (R * 8 + 63) / 64;
+  // the epilogue's operands are loaded first, so their latency overlaps the transforms
+  // (at a clamped pixel, unconditionally: a load under a lane-dependent branch
+  // would be waited for at the branch's end)
+  const bool pixel = threadIdx.x < a.n0;
+  const size_t pix = static_cast<size_t>(n1) * a.n0 + min(threadIdx.x, a.n0 - 1);
+  // (an absent input reads the output column instead, unused)
+  const float apv = a.apod_t[pix];
+  const float2 regv = (a.reg_in ? a.reg_in : a.image_out)[pix];
+  const float2 rv = (a.dot_r ? a.dot_r : a.image_out)[pix];
   float2 acc = make_float2(0.f, 0.f);
   const uint32_t chunks = (a.coils + 7) / 8;
   for (uint32_t chunk = 0; chunk < chunks; ++chunk) {
     const uint32_t cw = min(8u, a.coils - chunk * 8);
+    // conj(S_c[n0]) of the wave's outputs, loaded with the chunk's data
+    const uint32_t c = chunk * 8 + w;
+    const float2* sm = a.smaps_t + (static_cast<size_t>(n1) * a.cstride + min(c, a.coils - 1)) * a.n0;
+    float2 smv[TPL][4];
+#pragma unroll
+    for (int s = 0; s < TPL; ++s) {
+      const int task = lane + 64 * s;
+      const int k1 = task >> 3, k2a = task & 7;
+#pragma unroll
+      for (int k2b = 0; k2b < 4; ++k2b)
+        smv[s][k2b] = task < R * 8 ? sm[k1 + R * (k2a + 8 * k2b)] : make_float2(0.f, 0.f);
+    }
     fill_exchange<K>(a.t_inv + static_cast<size_t>(n1) * K * a.cstride + chunk * 8, a.cstride, cw, lds);
     __syncthreads();
     float2 x[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) x[j] = lds[xslot(lane + 64 * j, w)];
     __syncthreads();
-    constexpr int TPL = (R * 8 + 63) / 64;
     float2 out[TPL][8];
     wave_fft<R, true, false, true>(x, lds + w * wave_lds_elems<R>(), a.tw0, twp, lane, out);
     // multiply by conj(S_c[n0]) before the coil sum
-    const uint32_t c = chunk * 8 + w;
-    const float2* sm = a.smaps_t + (static_cast<size_t>(n1) * a.cstride + min(c, a.coils - 1)) * a.n0;
 #pragma unroll
     for (int s = 0; s < TPL; ++s) {
-      const int task = lane + 64 * s;
-      if (task >= R * 8) continue;
-      const int k1 = task >> 3, k2a = task & 7;
+      if (lane + 64 * s >= R * 8) continue;
 #pragma unroll
-      for (int k2b = 0; k2b < 4; ++k2b) {
-        const int n0 = k1 + R * (k2a + 8 * k2b);
-        out[s][k2b] = (w < static_cast<int>(cw)) ? cmul_conj_b(out[s][k2b], sm[n0]) : make_float2(0.f, 0.f);
-      }
+      for (int k2b = 0; k2b < 4; ++k2b)
+        out[s][k2b] = (w < static_cast<int>(cw)) ? cmul_conj_b(out[s][k2b], smv[s][k2b]) : make_float2(0.f, 0.f);
     }
     __syncthreads();
     emit_exchange<R, true>(out, lds, lane, w);
@@ -244,19 +285,16 @@ __global__ __launch_bounds__(512) void inv_axis0_kernel(FftArgs args) {
     __syncthreads();
   }
   double part = 0.0, part_r = 0.0, part_v = 0.0;
-  if (threadIdx.x < a.n0) {
-    const size_t pix = static_cast<size_t>(n1) * a.n0 + threadIdx.x;
-    const float s = a.apod_t[pix] * a.out_scale;
+  if (pixel) {
+    const float s = apv * a.out_scale;
     float2 v = make_float2(acc.x * s, acc.y * s);
     if (a.reg_in) {
-      const float2 p = a.reg_in[pix];
-      v.x += a.lambda * p.x;
-      v.y += a.lambda * p.y;
-      part = static_cast<double>(p.x) * v.x + static_cast<double>(p.y) * v.y;
+      v.x += a.lambda * regv.x;
+      v.y += a.lambda * regv.y;
+      part = static_cast<double>(regv.x) * v.x + static_cast<double>(regv.y) * v.y;
     }
     if (a.dot_r) {
-      const float2 r = a.dot_r[pix];
-      part_r = static_cast<double>(r.x) * v.x + static_cast<double>(r.y) * v.y;
+      part_r = static_cast<double>(rv.x) * v.x + static_cast<double>(rv.y) * v.y;
       part_v = static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y;
     }
     a.image_out[pix] = v;
@@ -270,7 +308,8 @@ __global__ __launch_bounds__(512) void inv_axis0_kernel(FftArgs args) {
         a.dot_partial[2 * a.n1 + n1] = part_v;
       }
     }
-  }}
+  }
+}
 
 // ------------------------------------------------------------- launchers
 
