@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 import torch
 
@@ -60,9 +61,14 @@ class Direction:
         self.header = {name: int(getattr(h, name)) for name in HEADER_FIELDS}
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().tsn_direction_free(self._h)
-            self._h = None
+        # At interpreter exit the process is ending and module globals may
+        # already be gone: the memory is left to the process teardown.
+        h, self._h = getattr(self, "_h", None), None
+        try:
+            if h and not sys.is_finalizing():
+                lib().tsn_direction_free(h)
+        except Exception:
+            pass
 
     @property
     def handle(self) -> C.c_void_p:

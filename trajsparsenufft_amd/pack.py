@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 from dataclasses import dataclass
 
 import numpy as np
@@ -44,9 +45,14 @@ class HostPack:
         self._h = handle
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().tsn_host_pack_free(self._h)
-            self._h = None
+        # At interpreter exit the process is ending and module globals may
+        # already be gone: the memory is left to the process teardown.
+        h, self._h = getattr(self, "_h", None), None
+        try:
+            if h and not sys.is_finalizing():
+                lib().tsn_host_pack_free(h)
+        except Exception:
+            pass
 
     @property
     def handle(self) -> C.c_void_p:
@@ -105,9 +111,14 @@ class KbMatrix:
         self.numpoints = numpoints
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().tsn_kb_free(self._h)
-            self._h = None
+        # At interpreter exit the process is ending and module globals may
+        # already be gone: the memory is left to the process teardown.
+        h, self._h = getattr(self, "_h", None), None
+        try:
+            if h and not sys.is_finalizing():
+                lib().tsn_kb_free(h)
+        except Exception:
+            pass
 
     @property
     def handle(self) -> C.c_void_p:

@@ -13,6 +13,7 @@ raises ``TsnError`` with code 3, the reference's exit status 3.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 from dataclasses import dataclass
 
 import numpy as np
@@ -54,9 +55,14 @@ class Plan:
         self._graph_key = None
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().tsn_plan_free(self._h)
-            self._h = None
+        # At interpreter exit the process is ending and module globals may
+        # already be gone: the memory is left to the process teardown.
+        h, self._h = getattr(self, "_h", None), None
+        try:
+            if h and not sys.is_finalizing():
+                lib().tsn_plan_free(h)
+        except Exception:
+            pass
 
     @property
     def device_bytes(self) -> int:
@@ -206,9 +212,14 @@ class Batch:
         self._h = out
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().tsn_batch_free(self._h)
-            self._h = None
+        # At interpreter exit the process is ending and module globals may
+        # already be gone: the memory is left to the process teardown.
+        h, self._h = getattr(self, "_h", None), None
+        try:
+            if h and not sys.is_finalizing():
+                lib().tsn_batch_free(h)
+        except Exception:
+            pass
 
     @property
     def device_bytes(self) -> int:
