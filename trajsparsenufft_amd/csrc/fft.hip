@@ -130,6 +130,12 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
                 static_cast<uint64_t>(a.n1) * a.cstride, min(8u, a.coils - chunk * 8), a.cstride, lds);
 }
 
+// The second-axis passes take kRowsPerBlock consecutive grid rows per block:
+// the next row's loads are issued as soon as the current row is in LDS, so
+// they overlap its transform and stores, and the launch has half the blocks
+// to dispatch.
+constexpr uint32_t kRowsPerBlock = 2;
+
 // Forward, second axis: grid row k0 -> panel rows k0*K1 + k1.  OUT 0: FP16
 // hi/residual interleaved N16 panel blocks; OUT 1: float2 [row][coils];
 // OUT 2: BF16 hi/residual interleaved panel blocks.
@@ -141,55 +147,67 @@ __global__ __launch_bounds__(512) void fwd_axis1_kernel(FftArgs args) {
   extern __shared__ float2 lds[];
   constexpr int K = 64 * R;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint32_t k0 = blockIdx.x, chunk = blockIdx.y;
+  const uint32_t row0 = blockIdx.x * kRowsPerBlock, chunk = blockIdx.y;
   const uint32_t cw = min(8u, a.coils - chunk * 8);
-  // grid row k0 of T: N1 = K/2 rows (the plan checks grid = 2 x image)
-  fill_exchange<K / 2>(a.t_fwd + static_cast<size_t>(k0) * a.n1 * a.cstride + chunk * 8, a.cstride, cw, lds);
-  __syncthreads();
-  float2 x[R];
-#pragma unroll
-  for (int j = 0; j < R; ++j) {
-    const uint32_t n1 = lane + 64 * j;
-    x[j] = n1 < a.n1 ? lds[xslot(n1, w)] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  constexpr int TPL = (R * 8 + 63) / 64;
-  float2 out[TPL][8];
-  wave_fft<R, false, true, false>(x, lds + w * wave_lds_elems<R>(), a.tw1, twp, lane, out);
-  __syncthreads();
-  emit_exchange<R, false>(out, lds, lane, w);
-  __syncthreads();
   const uint64_t grid_rows = static_cast<uint64_t>(a.k0) * K;
-  if constexpr (OUT == 0 || OUT == 2) {
-    // four threads per panel row, one uint4 each (words 4q .. 4q + 3: Re for
-    // q < 2, else Im, of coils 4 (q & 1) .. + 3; hi | res << 16), so a wave's
-    // store covers 16 whole 64-byte rows
-    uint4* base = reinterpret_cast<uint4*>(a.panel_il + (chunk * grid_rows + static_cast<uint64_t>(k0) * K) * 16);
-    for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(K) * 4; e += blockDim.x) {
-      const uint32_t k1 = e >> 2, q = e & 3;
-      uint32_t w4[4];
+  // grid row k0 of T: N1 = K/2 rows (the plan checks grid = 2 x image)
+  auto t_row = [&](uint32_t k0) { return a.t_fwd + static_cast<size_t>(k0) * a.n1 * a.cstride + chunk * 8; };
+  ExchangeRegs<K / 2> rows;
+  load_exchange<K / 2>(t_row(row0), a.cstride, cw, rows);
+  for (uint32_t i = 0; i < kRowsPerBlock; ++i) {
+    const uint32_t k0 = row0 + i;
+    if (k0 >= a.k0) break;
+    write_exchange<K / 2>(rows, lds);
+    __syncthreads();
+    if (i + 1 < kRowsPerBlock && k0 + 1 < a.k0) load_exchange<K / 2>(t_row(k0 + 1), a.cstride, cw, rows);
+    float2 x[R];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float2 v = lds[xslot(k1, 4 * (q & 1) + i)];
-        const float x = q < 2 ? v.x : v.y;
-        if constexpr (OUT == 2) {
-          w4[i] = split_bf16(x);
-        } else {
-          _Float16 h, r2;
-          split16(x, h, r2);
-          w4[i] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
-                  (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r2)) << 16);
-        }
-      }
-      base[static_cast<size_t>(k1) * 4 + q] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+    for (int j = 0; j < R; ++j) {
+      const uint32_t n1 = lane + 64 * j;
+      x[j] = n1 < a.n1 ? lds[xslot(n1, w)] : make_float2(0.f, 0.f);
     }
-  } else {
-    store_rows<K>(a.panel32 + static_cast<uint64_t>(k0) * K * a.cstride + chunk * 8, a.cstride, cw, a.cstride, lds);
+    __syncthreads();
+    constexpr int TPL = (R * 8 + 63) / 64;
+    float2 out[TPL][8];
+    wave_fft<R, false, true, false>(x, lds + w * wave_lds_elems<R>(), a.tw1, twp, lane, out);
+    __syncthreads();
+    emit_exchange<R, false>(out, lds, lane, w);
+    __syncthreads();
+    if constexpr (OUT == 0 || OUT == 2) {
+      // four threads per panel row, one uint4 each (words 4q .. 4q + 3: Re for
+      // q < 2, else Im, of coils 4 (q & 1) .. + 3; hi | res << 16), so a wave's
+      // store covers 16 whole 64-byte rows
+      uint4* base = reinterpret_cast<uint4*>(a.panel_il + (chunk * grid_rows + static_cast<uint64_t>(k0) * K) * 16);
+      for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(K) * 4; e += blockDim.x) {
+        const uint32_t k1 = e >> 2, q = e & 3;
+        uint32_t w4[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const float2 v = lds[xslot(k1, 4 * (q & 1) + t)];
+          const float x = q < 2 ? v.x : v.y;
+          if constexpr (OUT == 2) {
+            w4[t] = split_bf16(x);
+          } else {
+            _Float16 h, r2;
+            split16(x, h, r2);
+            w4[t] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
+                    (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r2)) << 16);
+          }
+        }
+        base[static_cast<size_t>(k1) * 4 + q] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+      }
+    } else {
+      store_rows<K>(a.panel32 + static_cast<uint64_t>(k0) * K * a.cstride + chunk * 8, a.cstride, cw, a.cstride,
+                    lds);
+    }
+    __syncthreads();  // the exchange area is refilled next
   }
 }
 
 // Inverse, second grid axis: grid row k0 ([k0*K1 + k1][coils]) -> IDFT along
-// k1, keep n1 < N1 -> T2[n1][k0][c].
+// k1, keep n1 < N1 -> T2[n1][k0][c].  A block's two rows are neighbours, so
+// its 64-byte rows of T2 fill whole 128-byte lines, and neighbouring pairs
+// sit on one XCD.
 template <int R>
 __global__ __launch_bounds__(512) void inv_axis1_kernel(FftArgs args) {
   const FftArgs a = problem_view(args, blockIdx.z);
@@ -198,23 +216,31 @@ __global__ __launch_bounds__(512) void inv_axis1_kernel(FftArgs args) {
   extern __shared__ float2 lds[];
   constexpr int K = 64 * R;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  // neighbouring grid rows on one XCD: their 64-byte rows of T2 fill whole L2 lines
-  const uint32_t k0 = xcd_remap(blockIdx.x, gridDim.x), chunk = blockIdx.y;
+  const uint32_t row0 = xcd_remap(blockIdx.x, gridDim.x) * kRowsPerBlock, chunk = blockIdx.y;
   const uint32_t cw = min(8u, a.coils - chunk * 8);
-  fill_exchange<K>(a.grid + static_cast<size_t>(k0) * K * a.cstride + chunk * 8, a.cstride, cw, lds);
-  __syncthreads();
-  float2 x[R];
+  auto g_row = [&](uint32_t k0) { return a.grid + static_cast<size_t>(k0) * K * a.cstride + chunk * 8; };
+  ExchangeRegs<K> rows;
+  load_exchange<K>(g_row(row0), a.cstride, cw, rows);
+  for (uint32_t i = 0; i < kRowsPerBlock; ++i) {
+    const uint32_t k0 = row0 + i;
+    if (k0 >= a.k0) break;
+    write_exchange<K>(rows, lds);
+    __syncthreads();
+    if (i + 1 < kRowsPerBlock && k0 + 1 < a.k0) load_exchange<K>(g_row(k0 + 1), a.cstride, cw, rows);
+    float2 x[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) x[j] = lds[xslot(lane + 64 * j, w)];
-  __syncthreads();
-  constexpr int TPL = (R * 8 + 63) / 64;
-  float2 out[TPL][8];
-  wave_fft<R, true, false, true>(x, lds + w * wave_lds_elems<R>(), a.tw1, twp, lane, out);
-  __syncthreads();
-  emit_exchange<R, true>(out, lds, lane, w);
-  __syncthreads();
-  store_rows<K / 2>(a.t_inv + static_cast<size_t>(k0) * a.cstride + chunk * 8,
-                    static_cast<uint64_t>(a.k0) * a.cstride, cw, a.cstride, lds);
+    for (int j = 0; j < R; ++j) x[j] = lds[xslot(lane + 64 * j, w)];
+    __syncthreads();
+    constexpr int TPL = (R * 8 + 63) / 64;
+    float2 out[TPL][8];
+    wave_fft<R, true, false, true>(x, lds + w * wave_lds_elems<R>(), a.tw1, twp, lane, out);
+    __syncthreads();
+    emit_exchange<R, true>(out, lds, lane, w);
+    __syncthreads();
+    store_rows<K / 2>(a.t_inv + static_cast<size_t>(k0) * a.cstride + chunk * 8,
+                      static_cast<uint64_t>(a.k0) * a.cstride, cw, a.cstride, lds);
+    __syncthreads();  // the exchange area is refilled next
+  }
 }
 
 // Inverse, first grid axis for image column n1: IDFT along k0, keep n0 < N0,
@@ -345,13 +371,13 @@ int fft_forward(const FftArgs& a, int out_kind, hipStream_t s, uint32_t problems
   TSN_DISPATCH_R(r0, hipLaunchKernelGGL(fwd_axis0_kernel<R>, dim3(a.n1, chunks, problems), dim3(512),
                                         lds_bytes_for<R>(), s, a));
   if (out_kind == 0) {
-    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 0>), dim3(a.k0, chunks, problems), dim3(512),
+    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 0>), dim3(div_up(a.k0, kRowsPerBlock), chunks, problems), dim3(512),
                                           lds_bytes_for<R>(), s, a));
   } else if (out_kind == 2) {
-    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 2>), dim3(a.k0, chunks, problems), dim3(512),
+    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 2>), dim3(div_up(a.k0, kRowsPerBlock), chunks, problems), dim3(512),
                                           lds_bytes_for<R>(), s, a));
   } else {
-    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 1>), dim3(a.k0, chunks, problems), dim3(512),
+    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 1>), dim3(div_up(a.k0, kRowsPerBlock), chunks, problems), dim3(512),
                                           lds_bytes_for<R>(), s, a));
   }
   TSN_HIP(hipGetLastError());
@@ -361,7 +387,7 @@ int fft_forward(const FftArgs& a, int out_kind, hipStream_t s, uint32_t problems
 int fft_inverse(const FftArgs& a, hipStream_t s, uint32_t problems) {
   const int r0 = radix_of(a.k0), r1 = radix_of(a.k1);
   const uint32_t chunks = (a.cstride + 7) / 8;
-  TSN_DISPATCH_R(r1, hipLaunchKernelGGL(inv_axis1_kernel<R>, dim3(a.k0, chunks, problems), dim3(512),
+  TSN_DISPATCH_R(r1, hipLaunchKernelGGL(inv_axis1_kernel<R>, dim3(div_up(a.k0, kRowsPerBlock), chunks, problems), dim3(512),
                                         lds_bytes_for<R>(), s, a));
   TSN_DISPATCH_R(r0, hipLaunchKernelGGL(inv_axis0_kernel<R>, dim3(a.n1, 1, problems), dim3(512),
                                         lds_bytes_for<R>(), s, a));

@@ -351,27 +351,48 @@ __device__ __forceinline__ void wave_fft(float2 (&x)[R], float2* buf, const floa
 // minimum 2-way bank sharing instead of up to 16-way conflicts.
 __device__ __forceinline__ int xslot(int k, int c) { return k * 8 + (c ^ ((k ^ (k >> 3)) & 7)); }
 
-// Fill the exchange area with ROWS rows x 8 coils from global memory (row r,
-// coil c at src[r * row_stride + c]; coils >= cw read as zero).  All loads are
-// issued before the first LDS write, so a block pays one memory latency, not
-// one per loop trip.  512-thread blocks.
+// ROWS rows x 8 coils of global memory held in registers on their way to the
+// exchange area (512-thread blocks): thread t holds elements t + 512 i, row
+// e >> 3, coil e & 7.
+template <int ROWS>
+struct ExchangeRegs {
+  float2 v[(ROWS * 8 + 511) / 512];
+};
+
+// Issue the loads of ROWS rows (row r, coil c at src[r * row_stride + c];
+// coils >= cw read as zero) without waiting for them.
+template <int ROWS>
+__device__ __forceinline__ void load_exchange(const float2* __restrict__ src, uint64_t row_stride, uint32_t cw,
+                                              ExchangeRegs<ROWS>& r) {
+  constexpr int kElems = ROWS * 8;
+#pragma unroll
+  for (int i = 0; i < (kElems + 511) / 512; ++i) {
+    const uint32_t e = threadIdx.x + 512u * i;
+    const uint32_t row = e >> 3, c = e & 7;
+    r.v[i] = (e < static_cast<uint32_t>(kElems) && c < cw) ? src[row * row_stride + c] : make_float2(0.f, 0.f);
+  }
+}
+
+// Write loaded rows to the exchange area.
+template <int ROWS>
+__device__ __forceinline__ void write_exchange(const ExchangeRegs<ROWS>& r, float2* lds) {
+  constexpr int kElems = ROWS * 8;
+#pragma unroll
+  for (int i = 0; i < (kElems + 511) / 512; ++i) {
+    const uint32_t e = threadIdx.x + 512u * i;
+    if (e < static_cast<uint32_t>(kElems)) lds[xslot(static_cast<int>(e >> 3), static_cast<int>(e & 7))] = r.v[i];
+  }
+}
+
+// Fill the exchange area with ROWS rows x 8 coils from global memory: all
+// loads are issued before the first LDS write, so a block pays one memory
+// latency, not one per loop trip.  512-thread blocks.
 template <int ROWS>
 __device__ __forceinline__ void fill_exchange(const float2* __restrict__ src, uint64_t row_stride, uint32_t cw,
                                               float2* lds) {
-  constexpr int kElems = ROWS * 8;
-  constexpr int kPer = (kElems + 511) / 512;
-  float2 v[kPer];
-#pragma unroll
-  for (int i = 0; i < kPer; ++i) {
-    const uint32_t e = threadIdx.x + 512u * i;
-    const uint32_t r = e >> 3, c = e & 7;
-    v[i] = (e < static_cast<uint32_t>(kElems) && c < cw) ? src[r * row_stride + c] : make_float2(0.f, 0.f);
-  }
-#pragma unroll
-  for (int i = 0; i < kPer; ++i) {
-    const uint32_t e = threadIdx.x + 512u * i;
-    if (e < static_cast<uint32_t>(kElems)) lds[xslot(static_cast<int>(e >> 3), static_cast<int>(e & 7))] = v[i];
-  }
+  ExchangeRegs<ROWS> r;
+  load_exchange<ROWS>(src, row_stride, cw, r);
+  write_exchange<ROWS>(r, lds);
 }
 
 // Store ROWS rows of the exchange area (8 coils each, coils >= cw skipped) to
