@@ -1106,9 +1106,12 @@ static int schedule_units(tsn_direction* d, uint32_t unit_tiles) {
     const int32_t gi = static_cast<int32_t>(g);
     if (e - b > kSplitAboveTiles) {  // a chunked long group
       flush();
-      for (int32_t t = b; t < e; t += kSplitAboveTiles) {
+      const uint32_t n = static_cast<uint32_t>(e - b + kSplitAboveTiles - 1) / kSplitAboveTiles;
+      for (uint32_t c = 0; c < n; ++c) {
         order.push_back(kChunkTag | chunk++);
-        chunk_tiles.push_back(std::min<int32_t>(kSplitAboveTiles, e - t));
+        // a chunk ranks as if longer by its finishing chain, so chunks start first
+        chunk_tiles.push_back(static_cast<int32_t>(chunk_begin(b, e, n, c + 1) - chunk_begin(b, e, n, c)) +
+                              kChunkFinishTiles);
       }
       continue;
     }
@@ -1163,12 +1166,9 @@ std::vector<tsn::ChunkItem> tsn::long_chunks(const std::vector<uint32_t>& off, u
     const uint32_t b = off[g], e = off[g + 1];
     if (e - b <= static_cast<uint32_t>(kSplitAboveTiles)) continue;
     const uint32_t n = (e - b + kSplitAboveTiles - 1) / kSplitAboveTiles;
-    for (uint32_t c = 0; c < n; ++c) {
-      const uint32_t cb = b + c * kSplitAboveTiles;
-      out.push_back({static_cast<int32_t>(g), static_cast<int32_t>(cb),
-                     static_cast<int32_t>(std::min<uint32_t>(e, cb + kSplitAboveTiles)), c, n, slot, counter,
-                     problem});
-    }
+    for (uint32_t c = 0; c < n; ++c)
+      out.push_back({static_cast<int32_t>(g), static_cast<int32_t>(chunk_begin(b, e, n, c)),
+                     static_cast<int32_t>(chunk_begin(b, e, n, c + 1)), c, n, slot, counter, problem});
     slot += n;
     ++counter;
   }

@@ -103,8 +103,18 @@ struct WorkItem {
   int32_t tile_end;
 };
 
-// One chunk of a long group: kSplitAboveTiles consecutive tiles of it (the
-// last chunk fewer).  Each chunk's wave publishes its partial sums to slot
+// Chunk c of the n = ceil(tiles / kSplitAboveTiles) chunks of a long group
+// [b, e): tiles [chunk_begin(c), chunk_begin(c + 1)), sizes within one of each
+// other, so no chunk is much longer than its group needs.
+inline uint32_t chunk_begin(uint32_t b, uint32_t e, uint32_t n, uint32_t c) {
+  return b + static_cast<uint32_t>((static_cast<uint64_t>(e - b) * c) / n);
+}
+// Launch-order rank of a chunk's finishing chain (publish, count, sum the
+// slots), in tiles: chunks sort ahead of units of their own length.
+constexpr int32_t kChunkFinishTiles = 4;
+
+// One chunk of a long group: consecutive tiles of it (chunk_begin).  Each
+// chunk's wave publishes its partial sums to slot
 // slot0 + chunk; the wave that brings the group's arrival counter to
 // n_chunks adds the slots in chunk order and writes the rows, so the sums do
 // not depend on the schedule or on which wave arrives last.
