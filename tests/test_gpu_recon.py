@@ -129,6 +129,28 @@ def test_cg_matches_fp64_oracle(precision):
                                atol=1e-4 * emu_hist[0])
 
 
+@pytest.mark.parametrize("iters", [1, 2, 3])
+def test_short_solves_match_model(iters):
+    """One to three CG steps: the first step starts from x = 0 without reading
+    x, r_0 shares p_0's buffer and the first update moves r to a buffer of its
+    own; every count matches the emulated solver and keeps its history."""
+
+    from oracle import nudft
+    from oracle.solver import cg_native
+    traj, packs, x, smaps, omega, y = _cg_case()
+    plan = _plan(packs, smaps, "split16")
+    scale = plan.operator_scale(dev(y))
+    res = plan.cg(dev(y), iterations=iters, operator_scale=scale, lam=1e-4)
+    torch.cuda.synchronize()
+    assert int(res.status.item()) == 1
+    model = _model(packs, smaps, "split16")
+    emu, emu_hist, _ = cg_native(model, y, iters, scale)
+    assert nudft.complex_rel_l2(res.image.cpu().numpy(), emu) <= 1e-4
+    hist = res.residuals.cpu().numpy()
+    assert hist.shape == (iters + 1,)
+    np.testing.assert_allclose(hist, np.asarray(emu_hist), rtol=1e-4)
+
+
 def test_graph_replay_is_bitwise_equal_to_eager():
     traj, packs, x, smaps, omega, y = _cg_case()
     plan = _plan(packs, smaps, "split16")

@@ -93,8 +93,8 @@ struct tsn_batch {
   float2 *tw0 = nullptr, *tw1 = nullptr, *smaps_t = nullptr, *t_fwd = nullptr, *t_inv = nullptr, *grid_buf = nullptr;
   float* apod_t = nullptr;
   uint32_t *pan_il = nullptr, *smp_il = nullptr, *coils_dev = nullptr, *m_dev = nullptr;
-  float2 *x = nullptr, *r = nullptr, *r2 = nullptr, *p = nullptr, *p2 = nullptr, *ap = nullptr, *b = nullptr;
-  double *partial = nullptr, *rr_partial = nullptr, *hist = nullptr;
+  float2 *x = nullptr, *r = nullptr, *r2 = nullptr, *p = nullptr, *p2 = nullptr, *ap = nullptr;
+  double *partial = nullptr, *hist = nullptr;
   uint32_t hist_cap = 0;
   int* status = nullptr;
   // per-call tables (device): measurement / output pointers and inverse-pass scales
@@ -188,39 +188,44 @@ int enqueue_batch_cg(tsn_batch* bt, uint32_t iterations, hipStream_t s) {
                        bt->m_dev, bt->bmax, norm, bt->smp_il, sample_stride(bt));
   for (const InterpBatch& ib : bt->ib_adj)
     if (ib.n_problems) TSN_CHECK(launch_interp_batch(ib, kind, 1, true, 1.0f, s));
+  // b written straight into p (r0 = p0 = b, x0 = 0), with the per-column
+  // partials of <b, b> in the <v, v> slot; the plan's schedule (plan.hip
+  // enqueue_cg) for every problem at once
+  const uint32_t cols = bt->cfg.image1;
   {
     FftArgs a = batch_fft_args(bt);
-    a.image_out = bt->b;
+    a.image_out = bt->p;
     a.scale_of = bt->scale_rhs;
+    a.dot_partial = bt->partial;
+    a.dot_r = bt->p;
     TSN_CHECK(fft_inverse(a, s, S));
   }
-  hipLaunchKernelGGL(cg_init_kernel, dim3(vb, S), dim3(kVecThreads), 0, s, bt->b, bt->x, bt->r, bt->p, n,
-                     bt->rr_partial, vs, vb);
-  hipLaunchKernelGGL(cg_rho_kernel, dim3(1, S), dim3(kVecThreads), 0, s, bt->rr_partial, vb, bt->hist, 0u, 0,
-                     bt->status, vb, bt->hist_cap);
-  // the plan's schedule (plan.hip enqueue_cg) for every problem at once
   float2* p_cur = bt->p;
   float2* p_other = bt->p2;
-  float2* r_cur = bt->r;
+  float2* r_cur = bt->p;
   float2* r_other = bt->r2;
   for (uint32_t it = 0; it < iterations; ++it) {
     FftArgs a = batch_fft_args(bt);
+    a.rho_hist = bt->hist;
+    a.cg_status = bt->status;
+    a.dot_count = cols;
     if (it == 0) {
       a.image_t = p_cur;
+      a.rho0_dots = bt->partial + 2 * cols;
     } else {
       a.cg_r = r_cur;
       a.cg_r_new = r_other;
       a.cg_x = bt->x;
+      a.x_zero = it == 1;
       a.cg_ap = bt->ap;
       a.cg_p_old = p_cur;
       a.cg_p_new = p_other;
       a.cg_dots = bt->partial;
-      a.dot_count = bt->cfg.image1;
-      a.cg_status = bt->status;
-      a.rho_hist = bt->hist;
       a.iter = it;
       std::swap(p_cur, p_other);
-      std::swap(r_cur, r_other);
+      float2* const r_done = r_cur;
+      r_cur = r_other;
+      r_other = it == 1 ? bt->r : r_done;  // r_0 was p_0's buffer, now p_other
     }
     TSN_CHECK(fft_forward(a, out_kind_of(bt->cfg.precision), s, S));
     for (const InterpBatch& ib : bt->ib_fwd)
@@ -236,8 +241,8 @@ int enqueue_batch_cg(tsn_batch* bt, uint32_t iterations, hipStream_t s) {
     TSN_CHECK(fft_inverse(inv, s, S));
   }
   hipLaunchKernelGGL(cg_final_kernel, dim3(vb, S), dim3(kVecThreads), 0, s, bt->x, p_cur, bt->partial,
-                     bt->cfg.image1, bt->hist, iterations - 1, bt->status, n, vs, 3 * bt->cfg.image1,
-                     bt->hist_cap);
+                     bt->cfg.image1, bt->hist, iterations - 1, iterations == 1 ? 1 : 0, bt->status, n, vs,
+                     3 * bt->cfg.image1, bt->hist_cap);
   TSN_HIP(hipGetLastError());
   return TSN_OK;
 }
@@ -357,9 +362,8 @@ int tsn_batch_create(const tsn_plan_config* cfg, const tsn_batch_problem* proble
   A(&bt->grid_buf, S * grid * bt->cmax);
   A(&bt->pan_il, S * panel_stride(bt));
   A(&bt->smp_il, S * sample_stride(bt));
-  for (float2** v : {&bt->x, &bt->r, &bt->r2, &bt->p, &bt->p2, &bt->ap, &bt->b}) A(v, S * px);
+  for (float2** v : {&bt->x, &bt->r, &bt->r2, &bt->p, &bt->p2, &bt->ap}) A(v, S * px);
   A(&bt->partial, S * 3 * static_cast<size_t>(c.image1));
-  A(&bt->rr_partial, S * bt->vec_blocks);
   A(&bt->status, S);
   A(&bt->coils_dev, S);
   A(&bt->m_dev, S);

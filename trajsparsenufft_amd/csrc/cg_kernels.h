@@ -15,58 +15,16 @@ using tsn::fail;
 
 constexpr int kVecThreads = 256;
 
-__device__ __forceinline__ double block_sum(double v, double* red) {
-  red[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-    if (static_cast<int>(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
-    __syncthreads();
-  }
-  return red[0];
-}
-
-// Fixed-order sum of partials: every block that calls it gets the same value.
-__device__ __forceinline__ double sum_partials(const double* __restrict__ partial, uint32_t count, double* red) {
-  double s = 0.0;
-  for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) s += partial[i];
-  return block_sum(s, red);
-}
-
-// x = 0, r = p = b, partial <r, r>
-__global__ __launch_bounds__(kVecThreads) void cg_init_kernel(const float2* __restrict__ b,
-                                                              float2* __restrict__ x, float2* __restrict__ r,
-                                                              float2* __restrict__ p, uint32_t n,
-                                                              double* __restrict__ partial, uint64_t vstride,
-                                                              uint32_t pstride) {
-  __shared__ double red[kVecThreads];
-  const uint64_t z = blockIdx.y;  // problem of a batched launch
-  b += z * vstride;
-  x += z * vstride;
-  r += z * vstride;
-  p += z * vstride;
-  partial += z * pstride;
-  double s = 0.0;
-  const uint32_t e = blockIdx.x * kVecThreads + threadIdx.x;
-  if (e < n) {
-    const float2 v = b[e];
-    x[e] = make_float2(0.f, 0.f);
-    r[e] = v;
-    p[e] = v;
-    s = static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y;
-  }
-  s = block_sum(s, red);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
-
-// The last CG step's update: alpha = rho / <p, Ap> from the inverse pass's
+// The last CG step's update (x taken as zero with x_zero: a one-step solve):
+// alpha = rho / <p, Ap> from the inverse pass's
 // partials dots [3][dot_count] (<p, Ap>, Re<r, Ap>, <Ap, Ap>; 0 and status 0
 // on non-positive or non-finite curvature), x += alpha p; block 0 writes
 // rho_hist[iter + 1] = ||r - alpha Ap||^2 (expanded as in the forward pass)
 // and turns rho_hist[0 .. iter + 1] into the residual history sqrt(rho).
 __global__ __launch_bounds__(kVecThreads) void cg_final_kernel(
     float2* __restrict__ x, const float2* __restrict__ p, const double* __restrict__ dots, uint32_t dot_count,
-    double* __restrict__ rho_hist, uint32_t iter, int* __restrict__ status, uint32_t n, uint64_t vstride,
-    uint32_t dstride, uint32_t hstride) {
+    double* __restrict__ rho_hist, uint32_t iter, int x_zero, int* __restrict__ status, uint32_t n,
+    uint64_t vstride, uint32_t dstride, uint32_t hstride) {
   __shared__ double red[kVecThreads];
   const uint64_t z = blockIdx.y;  // problem of a batched launch
   x += z * vstride;
@@ -83,7 +41,7 @@ __global__ __launch_bounds__(kVecThreads) void cg_final_kernel(
   const uint32_t e = blockIdx.x * kVecThreads + threadIdx.x;
   if (e < n) {
     const float2 pv = p[e];
-    float2 xv = x[e];
+    float2 xv = x_zero ? make_float2(0.f, 0.f) : x[e];
     xv.x = static_cast<float>(xv.x + alpha * pv.x);
     xv.y = static_cast<float>(xv.y + alpha * pv.y);
     x[e] = xv;
@@ -96,13 +54,14 @@ __global__ __launch_bounds__(kVecThreads) void cg_final_kernel(
 }
 
 // The last CG step fused with the result (single plans): alpha as in
-// cg_final_kernel, then x_out = transpose(x + alpha p), [cols][rows] from the
+// cg_final_kernel, then x_out = transpose(x + alpha p) (x taken as zero with
+// x_zero: a one-step solve), [cols][rows] from the
 // column-major [rows][cols] vectors, in 32 x 32 tiles through LDS (256
 // threads: 32 columns x 8 rows).  Block (0, 0) also writes rho_hist[iter + 1],
 // turns the history into sqrt(rho) and copies the status word to status_out.
 __global__ __launch_bounds__(kVecThreads) void cg_final_out_kernel(
     const float2* __restrict__ x, const float2* __restrict__ p, const double* __restrict__ dots,
-    uint32_t dot_count, double* __restrict__ rho_hist, uint32_t iter, int* __restrict__ status,
+    uint32_t dot_count, double* __restrict__ rho_hist, uint32_t iter, int x_zero, int* __restrict__ status,
     int* __restrict__ status_out, uint32_t rows, uint32_t cols, float2* __restrict__ x_out) {
   __shared__ double red[kVecThreads];
   __shared__ float2 tile[32][33];
@@ -114,7 +73,7 @@ __global__ __launch_bounds__(kVecThreads) void cg_final_out_kernel(
   for (int k = 0; k < 4; ++k) {
     const uint32_t r = by + ty + 8 * k, c = bx + tx;
     const bool in = r < rows && c < cols;
-    xv[k] = in ? x[static_cast<size_t>(r) * cols + c] : make_float2(0.f, 0.f);
+    xv[k] = in && !x_zero ? x[static_cast<size_t>(r) * cols + c] : make_float2(0.f, 0.f);
     pv[k] = in ? p[static_cast<size_t>(r) * cols + c] : make_float2(0.f, 0.f);
   }
   double pap, rap, apap;
@@ -141,28 +100,6 @@ __global__ __launch_bounds__(kVecThreads) void cg_final_out_kernel(
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i <= iter + 1; i += blockDim.x) rho_hist[i] = sqrt(fmax(rho_hist[i], 0.0));
-}
-
-// rho_hist[index] = sum of partials (status = 1 when index == 0).  With
-// finish, rho_hist[0 .. index] then become the residual history sqrt(rho).
-__global__ __launch_bounds__(kVecThreads) void cg_rho_kernel(const double* __restrict__ partial,
-                                                             uint32_t count, double* __restrict__ rho_hist,
-                                                             uint32_t index, int finish,
-                                                             int* __restrict__ status, uint32_t pstride,
-                                                             uint32_t hstride) {
-  __shared__ double red[kVecThreads];
-  const uint64_t z = blockIdx.y;  // problem of a batched launch
-  partial += z * pstride;
-  rho_hist += z * hstride;
-  status += z;
-  const double s = sum_partials(partial, count, red);
-  if (threadIdx.x == 0) {
-    rho_hist[index] = s;
-    if (index == 0) *status = 1;
-  }
-  if (!finish) return;
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i <= index; i += blockDim.x) rho_hist[i] = sqrt(fmax(rho_hist[i], 0.0));
 }
 
 // [rows][cols] -> [cols][rows] complex, 32x32 tiles, optional scale.

@@ -47,6 +47,8 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
     q0 = a.cg_dots[qi];
     q1 = a.cg_dots[a.dot_count + qi];
     q2 = a.cg_dots[2 * a.dot_count + qi];
+  } else if (a.rho0_dots) {
+    q0 = a.rho0_dots[min(threadIdx.x, a.dot_count - 1)];
   }
   const bool store_vec = chunk == 0 && w == 0;  // the wave that writes x, r and p_new
   // Loads are unconditional, at clamped indices (straight-line code lets every
@@ -58,7 +60,7 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
   const float2* src_p = a.cg_r ? a.cg_p_old : a.image_t;
   const float2* src_a = a.cg_r ? a.cg_ap : a.image_t;
   const float2* src_s = a.smaps_t + (static_cast<size_t>(n1) * a.cstride + min(c, a.coils - 1)) * a.n0;
-  const bool x_wave = a.cg_r && __builtin_amdgcn_readfirstlane(store_vec ? 1 : 0);
+  const bool x_wave = a.cg_r && !a.x_zero && __builtin_amdgcn_readfirstlane(store_vec ? 1 : 0);
   float2 img[R], sm[R], pv[R], av[R], xv[R];
   float ap[R];
 #pragma unroll
@@ -80,6 +82,16 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
     if (!active || lane + 64u * j >= a.n0) {
       img[j] = pv[j] = sm[j] = av[j] = xv[j] = make_float2(0.f, 0.f);
       ap[j] = 0.f;
+    }
+  }
+  if (!a.cg_r && a.rho0_dots) {
+    // rho0 = <b, b>: the same fixed-order sum in every block, written by block (0, 0)
+    if (threadIdx.x >= a.dot_count) q0 = 0.0;
+    for (uint32_t i = threadIdx.x + 512; i < a.dot_count; i += 512) q0 += a.rho0_dots[i];
+    block_sum3<8>(q0, q1, q2, red);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+      a.rho_hist[0] = q0;
+      *a.cg_status = 1;
     }
   }
   if (a.cg_r) {

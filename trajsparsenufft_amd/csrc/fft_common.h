@@ -51,6 +51,11 @@ struct FftArgs {
   int* cg_status;
   double* rho_hist;
   uint32_t iter;
+  uint32_t x_zero;           // x is still zero (the first update): x = alpha p_old, x not read
+  // First forward pass of a solve (cg_r null): rho0_dots [dot_count] holds the
+  // partials of <b, b>; block (0, 0) writes their fixed-order sum to
+  // rho_hist[0] and 1 to *cg_status.  Null: nothing.
+  const double* rho0_dots;
   // Batched launches (blockIdx.z = problem z): every per-problem buffer sits
   // z strides (elements) from the pointers above; apod_t and the twiddles are
   // shared.  A single problem leaves them 0.
@@ -61,7 +66,7 @@ struct FftArgs {
   uint64_t t_stride;         // t_fwd, t_inv
   uint64_t grid_stride;      // grid, panel32
   uint64_t panel_stride;     // panel_il (uint32 words)
-  uint64_t partial_stride;   // dot_partial, cg_dots
+  uint64_t partial_stride;   // dot_partial, cg_dots, rho0_dots
   uint64_t hist_stride;      // rho_hist (cg_status: one int per problem)
 };
 
@@ -142,6 +147,7 @@ __device__ __forceinline__ FftArgs problem_view(const FftArgs& a, uint32_t z) {
   v.dot_partial = a.dot_partial ? a.dot_partial + zi * a.partial_stride : nullptr;
   v.cg_dots = a.cg_dots ? a.cg_dots + zi * a.partial_stride : nullptr;
   v.cg_status = a.cg_status ? a.cg_status + zi : nullptr;
+  v.rho0_dots = a.rho0_dots ? a.rho0_dots + zi * a.partial_stride : nullptr;
   v.rho_hist = a.rho_hist ? a.rho_hist + zi * a.hist_stride : nullptr;
   return v;
 }

@@ -51,10 +51,9 @@ struct tsn_plan {
   uint32_t *pan_il = nullptr, *smp_il = nullptr;  // interleaved hi | res << 16 panels
   float2* y_rows = nullptr;                         // measurement rows [M][coils] (data consistency)
   float2 *pan32 = nullptr, *smp32 = nullptr, *grid = nullptr;
-  float2 *x = nullptr, *r = nullptr, *r2 = nullptr, *p = nullptr, *p2 = nullptr, *ap = nullptr, *b = nullptr,
-         *tmp = nullptr;
-  // [3][image1] partials of <p, Ap>, Re<r, Ap>, <Ap, Ap> per image column; <r, r> per vector block
-  double *partial = nullptr, *rr_partial = nullptr;
+  float2 *x = nullptr, *r = nullptr, *r2 = nullptr, *p = nullptr, *p2 = nullptr, *ap = nullptr, *tmp = nullptr;
+  // [3][image1] partials of <p, Ap>, Re<r, Ap>, <Ap, Ap> per image column (first: <b, b> in the third)
+  double* partial = nullptr;
   int* status = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -188,43 +187,46 @@ int enqueue_cg(tsn_plan* pl, const float2* measurement, uint32_t iterations, flo
   const float inv_scale_sq = static_cast<float>(1.0 / sq);
   const double kk = static_cast<double>(pl->cfg.grid0) * pl->cfg.grid1;
   const float scaled_lambda = static_cast<float>(lambda / kk / sq);
-  const uint32_t n = pl->pixels;
-  // b = A^H (y / sqrt(K0K1)) / s^2
+  // b = A^H (y / sqrt(K0K1)) / s^2, written straight into p (r0 = p0 = b,
+  // x0 = 0), with the per-column partials of <b, b> in the <v, v> slot
+  const uint32_t cols = pl->cfg.image1;
   TSN_CHECK(op_load_samples(pl, measurement, grid_norm(pl), s));
   TSN_CHECK(op_adjoint_grid(pl, s));
-  TSN_CHECK(op_adjoint_image(pl, pl->b, nullptr, 0.0f, inv_scale_sq, nullptr, s));
-  hipLaunchKernelGGL(cg_init_kernel, dim3(pl->vec_blocks), dim3(kVecThreads), 0, s, pl->b, pl->x, pl->r,
-                     pl->p, n, pl->rr_partial, uint64_t{0}, 0u);
-  hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kVecThreads), 0, s, pl->rr_partial, pl->vec_blocks, history,
-                     0u, 0, pl->status, 0u, 0u);
-  // rho_k lives in history[k] until the end.  Iteration k: the first forward
-  // FFT pass applies step k - 1 (alpha from the partials of the last inverse
-  // pass: x += alpha p, r -= alpha Ap, rho_k) and forms p_k = r_k + beta
-  // p_{k-1} (k > 0; r and p in ping-pong buffers); the last inverse pass forms
-  // A p_k and the partials of <p_k, A p_k>, Re<r_k, A p_k>, <A p_k, A p_k>.
-  // One kernel applies the last step.  Six kernels per iteration.
+  TSN_CHECK(op_adjoint_image(pl, pl->p, nullptr, 0.0f, inv_scale_sq, pl->partial, s, pl->p));
+  // rho_k lives in history[k] until the end.  The first forward FFT pass
+  // writes rho_0 = <b, b> and the status.  Iteration k > 0: the first forward
+  // pass applies step k - 1 (alpha from the partials of the last inverse pass:
+  // x += alpha p, r -= alpha Ap, rho_k) and forms p_k = r_k + beta p_{k-1}
+  // (r and p in ping-pong buffers; r_0 shares p_0's buffer, so the first
+  // update writes r into a third one); the last inverse pass forms A p_k and
+  // the partials of <p_k, A p_k>, Re<r_k, A p_k>, <A p_k, A p_k>.  One kernel
+  // applies the last step and writes the result.  Six kernels per iteration.
   float2* p_cur = pl->p;
   float2* p_other = pl->p2;
-  float2* r_cur = pl->r;
+  float2* r_cur = pl->p;
   float2* r_other = pl->r2;
   for (uint32_t it = 0; it < iterations; ++it) {
     FftArgs a = fft_args(pl);
+    a.rho_hist = history;
+    a.cg_status = pl->status;
+    a.dot_count = cols;
     if (it == 0) {
       a.image_t = p_cur;
+      a.rho0_dots = pl->partial + 2 * cols;
     } else {
       a.cg_r = r_cur;
       a.cg_r_new = r_other;
       a.cg_x = pl->x;
+      a.x_zero = it == 1;
       a.cg_ap = pl->ap;
       a.cg_p_old = p_cur;
       a.cg_p_new = p_other;
       a.cg_dots = pl->partial;
-      a.dot_count = pl->cfg.image1;
-      a.cg_status = pl->status;
-      a.rho_hist = history;
       a.iter = it;
       std::swap(p_cur, p_other);
-      std::swap(r_cur, r_other);
+      float2* const r_done = r_cur;
+      r_cur = r_other;
+      r_other = it == 1 ? pl->r : r_done;  // r_0 was p_0's buffer, now p_other
     }
     TSN_CHECK(fft_forward(a, fft_out_kind(pl), s));
     TSN_CHECK(op_forward_samples(pl, nullptr, s));
@@ -233,8 +235,8 @@ int enqueue_cg(tsn_plan* pl, const float2* measurement, uint32_t iterations, flo
   }
   // the last step, x_out = transpose(x) and the status copy in one kernel
   hipLaunchKernelGGL(cg_final_out_kernel, dim3(div_up(pl->cfg.image0, 32), div_up(pl->cfg.image1, 32)),
-                     dim3(kVecThreads), 0, s, pl->x, p_cur, pl->partial, pl->cfg.image1, history, iterations - 1,
-                     pl->status, status_out, pl->cfg.image1, pl->cfg.image0, x_out);
+                     dim3(kVecThreads), 0, s, pl->x, p_cur, pl->partial, cols, history, iterations - 1,
+                     iterations == 1 ? 1 : 0, pl->status, status_out, pl->cfg.image1, pl->cfg.image0, x_out);
   TSN_HIP(hipGetLastError());
   return TSN_OK;
 }
@@ -317,9 +319,8 @@ int tsn_plan_create(const tsn_plan_config* cfg, const tsn_direction* fwd, const 
     A(&pl->smp_il, static_cast<size_t>(pl->blocks) * pl->m * kN);
     A(&pl->y_rows, static_cast<size_t>(pl->m) * c.coils);
   }
-  for (float2** v : {&pl->x, &pl->r, &pl->r2, &pl->p, &pl->p2, &pl->ap, &pl->b, &pl->tmp}) A(v, px);
+  for (float2** v : {&pl->x, &pl->r, &pl->r2, &pl->p, &pl->p2, &pl->ap, &pl->tmp}) A(v, px);
   A(&pl->partial, 3 * static_cast<size_t>(c.image1));
-  A(&pl->rr_partial, pl->vec_blocks);
   A(&pl->status, 1);
   if (rc == TSN_OK) {
     const auto t0 = twiddles(c.grid0), t1 = twiddles(c.grid1);
