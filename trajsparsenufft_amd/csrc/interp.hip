@@ -1275,6 +1275,8 @@ static int upload_pack(const HostPack& p, tsn_direction** out) {
   return TSN_OK;
 }
 
+constexpr uint32_t kMemberGroup = 8;
+
 int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* out) {
   *out = InterpBatch{};
   if (parts.empty()) return fail(TSN_ERR_ARGUMENT, "empty interpolation batch");
@@ -1309,8 +1311,10 @@ int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* o
   // is split at its XCD boundaries, and XCD x of the batch runs piece x of
   // every direction, one after the other: as in a launch of its own, each
   // XCD's L2 holds one eighth of a panel's columns at a time.  Within a
-  // piece the members of a direction (coil blocks, frames) follow each
-  // entry, so they reuse its tile operands from L2.
+  // piece, groups of kMemberGroup members of a direction (frames sharing a
+  // pack) follow each entry, so they reuse its tile operands from L2 while
+  // only that many panels are gathered from at once (24 frames of 768^2 / 16
+  // coils: 243 ms with all members per entry, 231 ms in groups of 8).
   std::vector<const tsn_direction*> dirs;
   for (const InterpBatchPart& p : parts)
     if (std::find(dirs.begin(), dirs.end(), p.dir) == dirs.end()) dirs.push_back(p.dir);
@@ -1329,9 +1333,11 @@ int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* o
     const uint32_t nd = static_cast<uint32_t>(d->host_order.size());
     for (uint32_t x = 0; x < 8; ++x) {
       const uint32_t lo = xcd_remap(x, nd), hi = x == 7 ? nd : xcd_remap(x + 1, nd);
+      const uint32_t nm = static_cast<uint32_t>(members.size());
+      for (uint32_t k0 = 0; k0 < nm; k0 += kMemberGroup)
       for (uint32_t i = lo; i < hi; ++i) {
         const uint32_t e = d->host_order[i];
-        for (uint32_t k = 0; k < members.size(); ++k) {
+        for (uint32_t k = k0; k < std::min(nm, k0 + kMemberGroup); ++k) {
           if (e & kChunkTag) {
             ChunkItem c = dc[e & ~kChunkTag];
             c.slot0 += slots + k * ns;

@@ -150,8 +150,8 @@ struct InterpBatchPart {
 };
 
 // Device tables of a batched launch: XCD x runs piece x of every direction's
-// own launch order, and the parts sharing a direction follow each entry (so
-// its tiles are read from HBM once and then from cache).
+// own launch order, and groups of the parts sharing a direction follow each
+// entry (so its tiles are read from HBM once per group, then from cache).
 struct InterpBatch {
   void* problems = nullptr;
   WorkItem* items = nullptr;
