@@ -265,13 +265,14 @@ constexpr int kStageWords = kSplitAboveTiles * kK;  // per wave
 
 template <int NB>
 __device__ __forceinline__ void stage_cols(const TileRsrc<NB>& r, int first, int n, uint32_t* stage, int lane) {
+  // both halves are loaded before either is stored (one memory round trip);
+  // past the last tile the buffer range check returns zeros
+  (void)n;
   first = __builtin_amdgcn_readfirstlane(first);
   const u32x4_t v0 = __builtin_amdgcn_raw_buffer_load_b128(r.cols, lane * 16, first * (kK * 4), 0);
+  const u32x4_t v1 = __builtin_amdgcn_raw_buffer_load_b128(r.cols, 1024 + lane * 16, first * (kK * 4), 0);
   reinterpret_cast<u32x4_t*>(stage)[lane] = v0;
-  if (__builtin_amdgcn_readfirstlane(n) > 8) {
-    const u32x4_t v1 = __builtin_amdgcn_raw_buffer_load_b128(r.cols, 1024 + lane * 16, first * (kK * 4), 0);
-    reinterpret_cast<u32x4_t*>(stage)[64 + lane] = v1;
-  }
+  reinterpret_cast<u32x4_t*>(stage)[64 + lane] = v1;
   __builtin_amdgcn_wave_barrier();
 }
 
