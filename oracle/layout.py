@@ -594,7 +594,7 @@ def compress_tile(values: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
 
 def bf16_bits(values: np.ndarray) -> np.ndarray:
     """uint16 bfloat16 bits of float32(values), round to nearest even (NaN kept
-    quiet).  Not a reference function: the BF16 tile variant of this build."""
+    quiet).  Not a reference function: the library's BF16 tile variant."""
 
     u = np.asarray(values, dtype=np.float32).view(np.uint32).astype(np.uint64)
     nan = (u & 0x7FFFFFFF) > 0x7F800000

@@ -101,7 +101,7 @@ def apply_csr_adjoint(cols: np.ndarray, re: np.ndarray, im: np.ndarray, y: np.nd
 
 def components(values: np.ndarray, mode: str) -> list[np.ndarray]:
     """dense_components / sparse_components value rounding (float32 base);
-    "bf16" / "bf16split" are this build's BF16 variant (round to nearest even)."""
+    "bf16" / "bf16split" are the library's BF16 variant (round to nearest even)."""
 
     v = np.asarray(values, dtype=np.float32)
     if mode == "fp32":
