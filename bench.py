@@ -74,7 +74,7 @@ def time_once(torch, fn, reps):
 
 
 def batch_workloads(torch, log, reps=3):
-    """The two batched workloads of the build target, each one hipGraph:
+    """The two batched workloads, each one hipGraph:
 
     * slices: 48 independent 256^2 / 512^2 slices, spoke counts uniform in
       64..402 (x 512 samples), coils from {4, 8, 12, 16}, each with its own
