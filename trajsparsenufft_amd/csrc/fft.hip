@@ -20,6 +20,16 @@
 
 namespace tsn {
 
+// Stand-in source of the CG partials for passes that have none (read, never written).
+__device__ const double kZeroPartials[1] = {0.0};
+
+// Global-space view of a pointer chosen between sources: a generic (flat)
+// load would count on the LDS wait counter too and force full waits.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const T* p) {
+  return (const __attribute__((address_space(1))) T*)(p);
+}
+
 // ---------------------------------------------------------------- kernels
 
 // Forward, first axis: image column n1 (all n0 < N0) of x*S_c*apod for eight
@@ -40,16 +50,20 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
   // issue every load first: the previous step's partials and rho (needed
   // first), then the image, or r, p, Ap and x, maps and apodization; the
   // reduction of the CG scalars then overlaps the latency of the later loads
-  double q0 = 0.0, q1 = 0.0, q2 = 0.0, rho_old = 0.0;
-  if (a.cg_r) {  // (threads past dot_count load its last entry and drop it below)
-    rho_old = a.rho_hist[a.iter - 1];
-    const uint32_t qi = min(threadIdx.x, a.dot_count - 1);
-    q0 = a.cg_dots[qi];
-    q1 = a.cg_dots[a.dot_count + qi];
-    q2 = a.cg_dots[2 * a.dot_count + qi];
-  } else if (a.rho0_dots) {
-    q0 = a.rho0_dots[min(threadIdx.x, a.dot_count - 1)];
-  }
+  // (branch-free: selected sources, a zero array for the absent ones, so no
+  // wait is forced where branches would merge; threads past dot_count load
+  // its last entry and drop it below)
+  const bool cg = a.cg_r != nullptr;
+  const bool sums = cg || a.rho0_dots != nullptr;
+  const uint32_t qi = sums ? min(threadIdx.x, a.dot_count - 1) : 0u;
+  const uint32_t qs = cg ? a.dot_count : 0u;
+  const double* q_src = cg ? a.cg_dots : (a.rho0_dots ? a.rho0_dots : kZeroPartials);
+  const double* q_more = cg ? a.cg_dots : kZeroPartials;
+  double q0 = gmem(q_src)[qi];
+  double q1 = gmem(q_more)[qs + (cg ? qi : 0u)];
+  double q2 = gmem(q_more)[2 * qs + (cg ? qi : 0u)];
+  const double rho_old = *gmem(cg ? a.rho_hist + (a.iter - 1) : kZeroPartials);
+  __builtin_amdgcn_sched_barrier(0);  // keep these loads ahead of the data loads
   const bool store_vec = chunk == 0 && w == 0;  // the wave that writes x, r and p_new
   // Loads are unconditional, at clamped indices (straight-line code lets every
   // load issue before the first wait; a load under a lane-dependent branch
@@ -77,6 +91,24 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
     ap[j] = a.apod_t[pix];
     if (x_wave) xv[j] = a.cg_x[pix];
   }
+  // the partials' sums (every block, same order), while the data loads are in
+  // flight: the previous step's <p, Ap>, Re<r, Ap>, <Ap, Ap>, or rho0 = <b, b>
+  // (written by block (0, 0) with the status)
+  if (sums) {
+    if (threadIdx.x >= a.dot_count) q0 = q1 = q2 = 0.0;
+    for (uint32_t i = threadIdx.x + 512; i < a.dot_count; i += 512) {
+      q0 += gmem(q_src)[i];
+      if (cg) {
+        q1 += gmem(a.cg_dots)[a.dot_count + i];
+        q2 += gmem(a.cg_dots)[2 * a.dot_count + i];
+      }
+    }
+    block_sum3<8>(q0, q1, q2, red);
+    if (!cg && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+      a.rho_hist[0] = q0;
+      *a.cg_status = 1;
+    }
+  }
 #pragma unroll
   for (int j = 0; j < JN; ++j) {
     if (!active || lane + 64u * j >= a.n0) {
@@ -84,25 +116,7 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
       ap[j] = 0.f;
     }
   }
-  if (!a.cg_r && a.rho0_dots) {
-    // rho0 = <b, b>: the same fixed-order sum in every block, written by block (0, 0)
-    if (threadIdx.x >= a.dot_count) q0 = 0.0;
-    for (uint32_t i = threadIdx.x + 512; i < a.dot_count; i += 512) q0 += a.rho0_dots[i];
-    block_sum3<8>(q0, q1, q2, red);
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-      a.rho_hist[0] = q0;
-      *a.cg_status = 1;
-    }
-  }
-  if (a.cg_r) {
-    // the previous step's scalars: every block reduces the same partials in the same order
-    if (threadIdx.x >= a.dot_count) q0 = q1 = q2 = 0.0;
-    for (uint32_t i = threadIdx.x + 512; i < a.dot_count; i += 512) {
-      q0 += a.cg_dots[i];
-      q1 += a.cg_dots[a.dot_count + i];
-      q2 += a.cg_dots[2 * a.dot_count + i];
-    }
-    block_sum3<8>(q0, q1, q2, red);
+  if (cg) {
     const double pap = q0, rap = q1, apap = q2;
     const bool first_block = blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0;
     double alpha = 0.0;
