@@ -150,7 +150,7 @@ int tsn_direction_schedule(tsn_direction* dir, uint32_t unit_tiles);
 /* Tile budget of the current schedule (0 for CSR directions). */
 uint32_t tsn_direction_unit_tiles(const tsn_direction* dir);
 /* Autotuner: time the interpolation of this direction for `coils` coils over
- * the candidate unit sizes {2,3,4,5,6,8,10,12,16} (scratch panels allocated
+ * the candidate unit sizes {2,3,4,5,6,8,10,12,14,16} (scratch panels allocated
  * and freed here, the stream synchronized) and keep the fastest schedule.
  * Results do not depend on the schedule (bitwise).  No counterpart in the
  * reference; run once per trajectory and coil count. */

@@ -1487,7 +1487,7 @@ int tsn_direction_autotune(tsn_direction* d, uint32_t coils, void* stream, uint3
   hip(hipEventCreate(&e0), "event");
   hip(hipEventCreate(&e1), "event");
   const int kind = d->values == TSN_VALUES_BF16 ? kSparseSplitBF16 : kSparseSplit;
-  const uint32_t candidates[] = {2, 3, 4, 5, 6, 8, 10, 12, 16};
+  const uint32_t candidates[] = {2, 3, 4, 5, 6, 8, 10, 12, 14, 16};
   uint32_t best = 0;
   float best_ms = 0.0f;
   for (uint32_t u : candidates) {
