@@ -156,11 +156,26 @@ uint32_t tsn_direction_unit_tiles(const tsn_direction* dir);
  * reference; run once per trajectory and coil count. */
 int tsn_direction_autotune(tsn_direction* dir, uint32_t coils, void* stream, uint32_t* best_unit_tiles,
                            float* best_us);
-/* Device bytes the direction occupies (pack + index arrays). */
+/* Device bytes the direction occupies (pack + index arrays, plus the
+ * dense-control tiles once they have been expanded). */
 size_t tsn_direction_bytes(const tsn_direction* dir);
+/* Expand the dense-control tiles (16x32 FP16, 2 KiB per tile) from the
+ * direction's sparse operands on the device.  Uploads leave them out; dense16
+ * plans, batches and tsn_interp_dense expand them on first use.  Idempotent;
+ * synchronous.  Counterpart of experiments/packing/prepare_dense_control.py:27-87
+ * (decode_row_metadata / expand_component), which writes them offline. */
+int tsn_direction_expand_dense(tsn_direction* dir);
 void tsn_direction_free(tsn_direction* dir);
 
-/* Interpolation y = G x (conjugate = 0) or G^H-style conjugate combine (1)
+/* Concurrency of the direct interpolation calls below (tsn_interp,
+ * tsn_interp_dense, tsn_interp_il, tsn_interp_dc): a long group (more than 16
+ * tiles) is summed by several waves through partial slots and an arrival
+ * counter that the direction holds, so launches of these calls on one
+ * direction must not overlap (one stream at a time, or ordered by events).
+ * Plans and batches hold their own slots and counters and may share a
+ * direction across streams.
+ *
+ * Interpolation y = G x (conjugate = 0) or G^H-style conjugate combine (1)
  * of FP16 N16 panels ([row][16] halves: Re coils 0..7, Im coils 0..7, blocks
  * of 8 coils, block stride n_cols*16 halves).  panel_res may be NULL (hi
  * only).  Output float2 per layout; accumulate adds to it.  scale multiplies
@@ -169,7 +184,7 @@ void tsn_direction_free(tsn_direction* dir);
 int tsn_interp(const tsn_direction* dir, const void* panel_hi, const void* panel_res, void* out,
                uint32_t coils, int conjugate, int accumulate, float scale, int out_layout,
                void* stream);
-/* Same operator on the dense-MFMA control (tiles expanded to 16x32 at upload). */
+/* Same operator on the dense-MFMA control (tiles expanded to 16x32 on first use). */
 int tsn_interp_dense(const tsn_direction* dir, const void* panel_hi, const void* panel_res,
                      void* out, uint32_t coils, int conjugate, int accumulate, float scale,
                      int out_layout, void* stream);

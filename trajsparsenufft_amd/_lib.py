@@ -69,6 +69,7 @@ SIGNATURES = {
     "tsn_direction_upload_csr": (C.c_int, [P, C.c_int, PP]),
     "tsn_direction_header": (C.c_int, [P, C.POINTER(PackHeader)]),
     "tsn_direction_bytes": (C.c_size_t, [P]),
+    "tsn_direction_expand_dense": (C.c_int, [P]),
     "tsn_direction_schedule": (C.c_int, [P, C.c_uint32]),
     "tsn_direction_unit_tiles": (C.c_uint32, [P]),
     "tsn_direction_autotune": (C.c_int, [P, C.c_uint32, P, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),

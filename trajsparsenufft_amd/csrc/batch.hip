@@ -9,6 +9,7 @@
 // Semantics per problem are exactly tsn_cg's (same kernels, same reduction
 // order), so a batched solve reproduces the single-plan solve bit for bit.
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include "cg_kernels.h"
@@ -60,7 +61,9 @@ __global__ __launch_bounds__(256) void batch_split_kernel(const float2* const* _
   (void)blocks_max;
 }
 
-// x_z (column-major [n1][n0]) -> x_out[z] ([n0][n1]); history sqrt(rho) and status out.
+// x_z (column-major [n1][n0]) -> x_out[z] ([n0][n1]); history sqrt(rho) and
+// status out.  A launch of its own after cg_final_kernel, so every block of
+// that kernel has read rho before the history is read here.
 __global__ __launch_bounds__(256) void batch_out_kernel(const float2* __restrict__ x, uint64_t vstride,
                                                         uint32_t n0, uint32_t n1, float2* const* __restrict__ xout,
                                                         const double* __restrict__ hist, uint32_t hstride,
@@ -74,7 +77,7 @@ __global__ __launch_bounds__(256) void batch_out_kernel(const float2* __restrict
     xout[z][e] = x[z * vstride + static_cast<uint64_t>(i1) * n0 + i0];
   }
   if (blockIdx.x == 0) {
-    for (uint32_t i = threadIdx.x; i <= iterations; i += blockDim.x) hist_out[z][i] = hist[z * hstride + i];
+    for (uint32_t i = threadIdx.x; i <= iterations; i += blockDim.x) hist_out[z][i] = sqrt(fmax(hist[z * hstride + i], 0.0));
     if (threadIdx.x == 0) status_out[z] = status[z];
   }
 }
@@ -334,8 +337,11 @@ int tsn_batch_create(const tsn_plan_config* cfg, const tsn_batch_problem* proble
     else if (q.forward->h.n_cols != grid || q.adjoint->h.output_size != grid ||
              q.forward->h.output_size != q.adjoint->h.n_cols)
       rc = fail(TSN_ERR_INCOMPATIBLE, "problem %u: unexpected operator shape", z);
-    else if (c.precision == TSN_PREC_DENSE16 && (!q.forward->tile_dense || !q.adjoint->tile_dense))
-      rc = fail(TSN_ERR_ARGUMENT, "problem %u: dense control needs dense tiles", z);
+    else if (c.precision == TSN_PREC_DENSE16 &&
+             (ensure_dense_tiles(q.forward) != TSN_OK || ensure_dense_tiles(q.adjoint) != TSN_OK)) {
+      const std::string why = tsn_last_error();
+      rc = fail(TSN_ERR_ARGUMENT, "problem %u: dense-control tiles: %s", z, why.c_str());
+    }
     if (rc != TSN_OK) {
       batch_destroy(bt);
       return rc;

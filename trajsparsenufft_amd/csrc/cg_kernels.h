@@ -19,8 +19,10 @@ constexpr int kVecThreads = 256;
 // alpha = rho / <p, Ap> from the inverse pass's
 // partials dots [3][dot_count] (<p, Ap>, Re<r, Ap>, <Ap, Ap>; 0 and status 0
 // on non-positive or non-finite curvature), x += alpha p; block 0 writes
-// rho_hist[iter + 1] = ||r - alpha Ap||^2 (expanded as in the forward pass)
-// and turns rho_hist[0 .. iter + 1] into the residual history sqrt(rho).
+// rho_hist[iter + 1] = ||r - alpha Ap||^2 (expanded as in the forward pass).
+// Every block reads rho_hist[iter] and nothing else of the history, so the
+// one write cannot race with a read; the residual history sqrt(rho) is formed
+// by a later launch (batch_out_kernel).
 __global__ __launch_bounds__(kVecThreads) void cg_final_kernel(
     float2* __restrict__ x, const float2* __restrict__ p, const double* __restrict__ dots, uint32_t dot_count,
     double* __restrict__ rho_hist, uint32_t iter, int x_zero, int* __restrict__ status, uint32_t n,
@@ -46,23 +48,24 @@ __global__ __launch_bounds__(kVecThreads) void cg_final_kernel(
     xv.y = static_cast<float>(xv.y + alpha * pv.y);
     x[e] = xv;
   }
-  if (blockIdx.x != 0) return;
-  __syncthreads();
-  if (threadIdx.x == 0) rho_hist[iter + 1] = fmax(rho - 2.0 * alpha * rap + alpha * alpha * apap, 0.0);
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i <= iter + 1; i += blockDim.x) rho_hist[i] = sqrt(fmax(rho_hist[i], 0.0));
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    rho_hist[iter + 1] = fmax(rho - 2.0 * alpha * rap + alpha * alpha * apap, 0.0);
 }
 
 // The last CG step fused with the result (single plans): alpha as in
 // cg_final_kernel, then x_out = transpose(x + alpha p) (x taken as zero with
 // x_zero: a one-step solve), [cols][rows] from the
 // column-major [rows][cols] vectors, in 32 x 32 tiles through LDS (256
-// threads: 32 columns x 8 rows).  Block (0, 0) also writes rho_hist[iter + 1],
-// turns the history into sqrt(rho) and copies the status word to status_out.
+// threads: 32 columns x 8 rows).  Block (0, 0) also writes the residual
+// history hist_out[i] = sqrt(rho_i), i <= iter + 1 (rho_{iter+1} expanded as
+// in the forward pass), and copies the status word to status_out.  rho_hist
+// is only read here, and hist_out is a separate buffer, so no block can see a
+// value another block rewrote.
 __global__ __launch_bounds__(kVecThreads) void cg_final_out_kernel(
     const float2* __restrict__ x, const float2* __restrict__ p, const double* __restrict__ dots,
-    uint32_t dot_count, double* __restrict__ rho_hist, uint32_t iter, int x_zero, int* __restrict__ status,
-    int* __restrict__ status_out, uint32_t rows, uint32_t cols, float2* __restrict__ x_out) {
+    uint32_t dot_count, const double* __restrict__ rho_hist, uint32_t iter, int x_zero, int* __restrict__ status,
+    int* __restrict__ status_out, uint32_t rows, uint32_t cols, float2* __restrict__ x_out,
+    double* __restrict__ hist_out) {
   __shared__ double red[kVecThreads];
   __shared__ float2 tile[32][33];
   const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -81,8 +84,8 @@ __global__ __launch_bounds__(kVecThreads) void cg_final_out_kernel(
   const double rho = rho_hist[iter];
   const bool first = blockIdx.x == 0 && blockIdx.y == 0;
   double alpha = 0.0;
-  if (isfinite(pap) && pap > 0.0) alpha = rho / pap;
-  else if (first && threadIdx.x == 0) *status = 0;
+  const bool curved = isfinite(pap) && pap > 0.0;
+  if (curved) alpha = rho / pap;
 #pragma unroll
   for (int k = 0; k < 4; ++k)
     tile[ty + 8 * k][tx] = make_float2(static_cast<float>(xv[k].x + alpha * pv[k].x),
@@ -94,12 +97,12 @@ __global__ __launch_bounds__(kVecThreads) void cg_final_out_kernel(
     if (r < rows && c < cols) x_out[static_cast<size_t>(c) * rows + r] = tile[tx][ty + 8 * k];
   }
   if (!first) return;
+  for (uint32_t i = threadIdx.x; i <= iter; i += blockDim.x) hist_out[i] = sqrt(fmax(rho_hist[i], 0.0));
   if (threadIdx.x == 0) {
-    rho_hist[iter + 1] = fmax(rho - 2.0 * alpha * rap + alpha * alpha * apap, 0.0);
+    hist_out[iter + 1] = sqrt(fmax(rho - 2.0 * alpha * rap + alpha * alpha * apap, 0.0));
+    if (!curved) *status = 0;
     *status_out = *status;
   }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i <= iter + 1; i += blockDim.x) rho_hist[i] = sqrt(fmax(rho_hist[i], 0.0));
 }
 
 // [rows][cols] -> [cols][rows] complex, 32x32 tiles, optional scale.

@@ -828,7 +828,7 @@ static void launch_tiles(const InterpArgs& a, bool il, int om, bool conj, bool a
 static int launch_interp_impl(const tsn_direction* d, int kind, const void* hi, const void* res, uint32_t coils,
                               int out_mode, bool conj, bool accumulate, float scale, float* out_f32,
                               _Float16* out_hi, _Float16* out_res, hipStream_t stream, bool interleaved,
-                              uint32_t* out_il, const DcTerms* dc) {
+                              uint32_t* out_il, const DcTerms* dc, const ChunkScratch* scratch) {
   if (!d || d->csr) return fail(TSN_ERR_ARGUMENT, "interpolation needs a packed (non-CSR) direction");
   if (coils == 0 || coils > static_cast<uint32_t>(kMaxCoils))
     return fail(TSN_ERR_ARGUMENT, "coils must be in 1..16 (got %u)", coils);
@@ -851,8 +851,9 @@ static int launch_interp_impl(const tsn_direction* d, int kind, const void* hi, 
   a.chunks = d->chunks;
   a.n_chunks = d->n_chunks;
   a.order = d->order;
-  a.partials = d->partials;
-  a.counters = d->counters;
+  a.partials = scratch ? scratch->partials : d->partials;
+  a.counters = scratch ? scratch->counters : d->counters;
+  if (d->n_chunks && (!a.partials || !a.counters)) return fail(TSN_ERR_ARGUMENT, "missing chunk scratch");
   a.group_rows = d->group_rows;
   a.group_offsets = d->group_offsets;
   a.tile_cols = d->tile_cols;
@@ -893,16 +894,16 @@ static int launch_interp_impl(const tsn_direction* d, int kind, const void* hi, 
 int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* res, uint32_t coils,
                   int out_mode, bool conj, bool accumulate, float scale, float* out_f32,
                   _Float16* out_hi, _Float16* out_res, hipStream_t stream, bool interleaved,
-                  uint32_t* out_il) {
+                  uint32_t* out_il, const ChunkScratch* scratch) {
   return launch_interp_impl(d, kind, hi, res, coils, out_mode, conj, accumulate, scale, out_f32, out_hi, out_res,
-                            stream, interleaved, out_il, nullptr);
+                            stream, interleaved, out_il, nullptr, scratch);
 }
 
 int launch_interp_dc(const tsn_direction* d, int kind, const uint32_t* panel, uint32_t coils, float scale,
-                     const DcTerms& dc, uint32_t* out_il, hipStream_t s) {
+                     const DcTerms& dc, uint32_t* out_il, hipStream_t s, const ChunkScratch* scratch) {
   if (!dc.y) return fail(TSN_ERR_ARGUMENT, "missing measurements");
   return launch_interp_impl(d, kind, panel, nullptr, coils, kOutSplit16IL, false, false, scale, nullptr, nullptr,
-                            nullptr, s, true, out_il, &dc);
+                            nullptr, s, true, out_il, &dc, scratch);
 }
 
 int launch_dc_residual(const float2* pred, uint32_t rows, uint32_t coils, const DcTerms& dc, bool bf16,
@@ -1180,6 +1181,65 @@ std::vector<tsn::ChunkItem> tsn::long_chunks(const std::vector<uint32_t>& off, u
 
 namespace tsn {
 
+// Dense-control tiles from the sparse operands, one thread per (tile, lane):
+// lane r + 16q holds compressed row r, slots 4q .. 4q+3, i.e. the two kept
+// values of column chunks 2q and 2q+1; the index byte holds each chunk's
+// 4-bit code (positions lo = code & 3, hi = code >> 2 of its two values in
+// the chunk).  Out: expanded row r, columns 8q .. 8q+7 as {re 8 x f16},
+// {im 8 x f16}, zeros elsewhere (the kept value written second wins if a
+// code repeats a position, as in the reference's expansion).
+__global__ __launch_bounds__(256) void expand_dense_kernel(const uint4* __restrict__ a,
+                                                           const uint16_t* __restrict__ idx, uint64_t lanes,
+                                                           uint4* __restrict__ dense) {
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= lanes) return;
+  const uint4 v = a[e];
+  const uint32_t code = idx[e];
+  const uint32_t vals[2][4] = {{v.x & 0xFFFFu, v.x >> 16, v.y & 0xFFFFu, v.y >> 16},
+                               {v.z & 0xFFFFu, v.z >> 16, v.w & 0xFFFFu, v.w >> 16}};
+  uint4 out[2];
+#pragma unroll
+  for (int part = 0; part < 2; ++part) {  // re, im
+    uint16_t d[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t byte = (code >> (8 * part)) & 0xFFu;
+#pragma unroll
+    for (int cc = 0; cc < 2; ++cc) {
+      const uint32_t nib = (byte >> (4 * cc)) & 0xFu;
+      d[4 * cc + (nib & 3u)] = static_cast<uint16_t>(vals[part][2 * cc]);
+      d[4 * cc + (nib >> 2)] = static_cast<uint16_t>(vals[part][2 * cc + 1]);
+    }
+    out[part] = make_uint4(d[0] | (static_cast<uint32_t>(d[1]) << 16), d[2] | (static_cast<uint32_t>(d[3]) << 16),
+                           d[4] | (static_cast<uint32_t>(d[5]) << 16), d[6] | (static_cast<uint32_t>(d[7]) << 16));
+  }
+  dense[2 * e] = out[0];
+  dense[2 * e + 1] = out[1];
+}
+
+int ensure_dense_tiles(const tsn_direction* cd) {
+  if (!cd || cd->csr) return fail(TSN_ERR_ARGUMENT, "dense tiles need a packed (non-CSR) direction");
+  if (cd->tile_dense) return TSN_OK;
+  if (cd->h.tiles > kMaxDenseTiles)
+    return fail(TSN_ERR_ARGUMENT, "pack has %u tiles, dense-control tiles hold at most %u", cd->h.tiles,
+                kMaxDenseTiles);
+  // the dense tiles are a cache of the direction's own operands
+  auto* d = const_cast<tsn_direction*>(cd);
+  const uint64_t lanes = static_cast<uint64_t>(d->h.tiles) * 64;
+  if (lanes == 0) return TSN_OK;
+  uint4* dense = nullptr;
+  TSN_HIP(hipMalloc(reinterpret_cast<void**>(&dense), lanes * 2 * sizeof(uint4)));
+  hipLaunchKernelGGL(expand_dense_kernel, dim3(div_up(lanes, 256)), dim3(256), 0, 0, d->tile_a, d->tile_idx, lanes,
+                     dense);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    (void)hipFree(dense);
+    return fail(TSN_ERR_RUNTIME, "dense tile expansion: %s", hipGetErrorString(e));
+  }
+  d->tile_dense = dense;
+  d->bytes += lanes * 2 * sizeof(uint4);
+  return TSN_OK;
+}
+
 // Chunk table, partial slots and zeroed arrival counters of a direction's
 // long groups (fixed by the pack; rescheduling keeps them).
 static int upload_chunks(tsn_direction* d) {
@@ -1190,6 +1250,8 @@ static int upload_chunks(tsn_direction* d) {
   size_t bytes = 0;
   TSN_CHECK(device_copy(chunks.data(), chunks.size(), &d->chunks, &bytes));
   d->n_chunks = static_cast<uint32_t>(chunks.size());
+  d->n_slots = slots;
+  d->n_counters = counters;
   TSN_HIP(hipMalloc(reinterpret_cast<void**>(&d->partials), static_cast<size_t>(slots) * kSlotFloats * 4));
   TSN_HIP(hipMalloc(reinterpret_cast<void**>(&d->counters), counters * sizeof(uint32_t)));
   TSN_HIP(hipMemset(d->counters, 0, counters * sizeof(uint32_t)));
@@ -1198,12 +1260,13 @@ static int upload_chunks(tsn_direction* d) {
 }
 
 static int upload_pack(const HostPack& p, tsn_direction** out) {
-  TSN_CHECK(validate_pack(p));
   const tsn_pack_header& h = p.h;
+  if (h.tiles > kMaxDeviceTiles)
+    return fail(TSN_ERR_ARGUMENT, "pack has %u tiles, the device format holds at most %u", h.tiles, kMaxDeviceTiles);
+  TSN_CHECK(validate_pack(p));
   const size_t tiles = h.tiles;
   std::vector<uint4> tile_a(tiles * 64);
   std::vector<uint16_t> tile_idx(tiles * 64);
-  std::vector<uint4> tile_dense(tiles * 128);
   for (size_t t = 0; t < tiles; ++t) {
     uint32_t rm_re[16], rm_im[16];
     decode_row_meta(p.real_meta.data() + t * 16, rm_re);
@@ -1222,22 +1285,6 @@ static int upload_pack(const HostPack& p, tsn_direction** out) {
       tile_a[t * 64 + lane] = v;
       tile_idx[t * 64 + lane] = static_cast<uint16_t>(((rm_re[r] >> (8 * q)) & 0xFF) |
                                                       (((rm_im[r] >> (8 * q)) & 0xFF) << 8));
-      // dense control: expanded row r, columns 8q .. 8q+7 (chunks 2q, 2q+1)
-      uint16_t dre[8] = {0}, dim[8] = {0};
-      for (int cc = 0; cc < 2; ++cc) {
-        const int chunk = 2 * q + cc;
-        const uint32_t code_re = (rm_re[r] >> (4 * chunk)) & 0xF;
-        const uint32_t code_im = (rm_im[r] >> (4 * chunk)) & 0xF;
-        dre[4 * cc + (code_re & 3)] = cre[r * 16 + 2 * chunk];
-        dre[4 * cc + (code_re >> 2)] = cre[r * 16 + 2 * chunk + 1];
-        dim[4 * cc + (code_im & 3)] = cim[r * 16 + 2 * chunk];
-        dim[4 * cc + (code_im >> 2)] = cim[r * 16 + 2 * chunk + 1];
-      }
-      uint4 d0, d1;
-      std::memcpy(&d0, dre, 16);
-      std::memcpy(&d1, dim, 16);
-      tile_dense[(t * 64 + lane) * 2] = d0;
-      tile_dense[(t * 64 + lane) * 2 + 1] = d1;
     }
   }
   // Device column words (see TileCols): byte offset of the column's row in a
@@ -1264,7 +1311,6 @@ static int upload_pack(const HostPack& p, tsn_direction** out) {
   if (rc == TSN_OK) rc = device_copy(dev_cols.data(), dev_cols.size(), &d->tile_cols, &bytes);
   if (rc == TSN_OK) rc = device_copy(tile_a.data(), tile_a.size(), &d->tile_a, &bytes);
   if (rc == TSN_OK) rc = device_copy(tile_idx.data(), tile_idx.size(), &d->tile_idx, &bytes);
-  if (rc == TSN_OK) rc = device_copy(tile_dense.data(), tile_dense.size(), &d->tile_dense, &bytes);
   d->bytes = bytes;
   if (rc == TSN_OK) rc = schedule_units(d, 0);
   if (rc == TSN_OK) rc = upload_chunks(d);
@@ -1624,6 +1670,8 @@ int tsn_direction_header(const tsn_direction* d, tsn_pack_header* out) {
 
 size_t tsn_direction_bytes(const tsn_direction* d) { return d ? d->bytes : 0; }
 
+int tsn_direction_expand_dense(tsn_direction* d) { return ensure_dense_tiles(d); }
+
 void tsn_direction_free(tsn_direction* d) { release(d); }
 
 int tsn_interp(const tsn_direction* d, const void* hi, const void* res, void* out, uint32_t coils,
@@ -1641,6 +1689,7 @@ int tsn_interp_dense(const tsn_direction* d, const void* hi, const void* res, vo
   if (out_layout != TSN_OUT_COIL_MAJOR && out_layout != TSN_OUT_ROW_MAJOR)
     return fail(TSN_ERR_ARGUMENT, "unknown output layout %d", out_layout);
   if (!res) return fail(TSN_ERR_ARGUMENT, "dense control needs hi and residual panels");
+  TSN_CHECK(ensure_dense_tiles(d));
   return launch_interp(d, kDenseSplit, hi, res, coils, out_layout, conjugate != 0, accumulate != 0,
                        scale, static_cast<float*>(out), nullptr, nullptr,
                        static_cast<hipStream_t>(stream), false, nullptr);

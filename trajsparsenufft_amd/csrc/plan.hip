@@ -22,10 +22,6 @@
 #include "fft_common.h"
 
 namespace tsn {
-int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* res, uint32_t coils,
-                  int out_mode, bool conj, bool accumulate, float scale, float* out_f32,
-                  _Float16* out_hi, _Float16* out_res, hipStream_t stream, bool interleaved,
-                  uint32_t* out_il);
 int launch_split_panel_il(const float2* in, uint32_t rows, uint32_t coils, float scale, uint32_t* out,
                           hipStream_t stream, int values);
 int launch_interp_csr(const tsn_direction* d, const float2* in, uint32_t coils, int out_mode,
@@ -54,12 +50,23 @@ struct tsn_plan {
   float2 *x = nullptr, *r = nullptr, *r2 = nullptr, *p = nullptr, *p2 = nullptr, *ap = nullptr, *tmp = nullptr;
   // [3][image1] partials of <p, Ap>, Re<r, Ap>, <Ap, Ap> per image column (first: <b, b> in the third)
   double* partial = nullptr;
+  // chunk slots and counters of the two directions' long groups, the plan's
+  // own (so plans sharing a direction can run on different streams)
+  ChunkScratch fwd_chunks, adj_chunks;
+  // rho_k of the running solve (squared residual norms), f64[rho_cap]; the
+  // caller's history buffer only receives sqrt(rho_k) from the final kernel
+  double* rho = nullptr;
+  uint32_t rho_cap = 0;
   int* status = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
 
 namespace {
+
+// rho entries allocated with a plan (64 CG iterations); longer solves grow it
+constexpr uint32_t kRhoInitial = 65;
+constexpr uint32_t kMaxCgIterations = 1u << 20;
 
 template <typename T>
 int plan_alloc(tsn_plan* pl, T** ptr, size_t count) {
@@ -127,9 +134,10 @@ int op_forward_samples(tsn_plan* pl, float2* out_coil_major, hipStream_t s) {
   }
   if (out_coil_major)
     return launch_interp(pl->fwd, interp_kind(pl), pl->pan_il, nullptr, pl->cfg.coils, 0, false, false, scale,
-                         reinterpret_cast<float*>(out_coil_major), nullptr, nullptr, s, true, nullptr);
+                         reinterpret_cast<float*>(out_coil_major), nullptr, nullptr, s, true, nullptr,
+                         &pl->fwd_chunks);
   return launch_interp(pl->fwd, interp_kind(pl), pl->pan_il, nullptr, pl->cfg.coils, 3, false, false, scale,
-                       nullptr, nullptr, nullptr, s, true, pl->smp_il);
+                       nullptr, nullptr, nullptr, s, true, pl->smp_il, &pl->fwd_chunks);
 }
 
 // sample panels -> grid (conjugate transpose interpolation)
@@ -137,7 +145,7 @@ int op_adjoint_grid(tsn_plan* pl, hipStream_t s) {
   if (is_fp32(pl))
     return launch_interp_csr(pl->adj, pl->smp32, pl->cfg.coils, 1, true, false, 1.0f, pl->grid, s);
   return launch_interp(pl->adj, interp_kind(pl), pl->smp_il, nullptr, pl->cfg.coils, 1, true, false, 1.0f,
-                       reinterpret_cast<float*>(pl->grid), nullptr, nullptr, s, true, nullptr);
+                       reinterpret_cast<float*>(pl->grid), nullptr, nullptr, s, true, nullptr, &pl->adj_chunks);
 }
 
 // grid -> image (column-major) with combine, scale, regulariser, dot partial
@@ -193,7 +201,7 @@ int enqueue_cg(tsn_plan* pl, const float2* measurement, uint32_t iterations, flo
   TSN_CHECK(op_load_samples(pl, measurement, grid_norm(pl), s));
   TSN_CHECK(op_adjoint_grid(pl, s));
   TSN_CHECK(op_adjoint_image(pl, pl->p, nullptr, 0.0f, inv_scale_sq, pl->partial, s, pl->p));
-  // rho_k lives in history[k] until the end.  The first forward FFT pass
+  // rho_k lives in pl->rho[k]; the final kernel writes sqrt(rho_k) to history.  The first forward FFT pass
   // writes rho_0 = <b, b> and the status.  Iteration k > 0: the first forward
   // pass applies step k - 1 (alpha from the partials of the last inverse pass:
   // x += alpha p, r -= alpha Ap, rho_k) and forms p_k = r_k + beta p_{k-1}
@@ -207,7 +215,7 @@ int enqueue_cg(tsn_plan* pl, const float2* measurement, uint32_t iterations, flo
   float2* r_other = pl->r2;
   for (uint32_t it = 0; it < iterations; ++it) {
     FftArgs a = fft_args(pl);
-    a.rho_hist = history;
+    a.rho_hist = pl->rho;
     a.cg_status = pl->status;
     a.dot_count = cols;
     if (it == 0) {
@@ -235,8 +243,9 @@ int enqueue_cg(tsn_plan* pl, const float2* measurement, uint32_t iterations, flo
   }
   // the last step, x_out = transpose(x) and the status copy in one kernel
   hipLaunchKernelGGL(cg_final_out_kernel, dim3(div_up(pl->cfg.image0, 32), div_up(pl->cfg.image1, 32)),
-                     dim3(kVecThreads), 0, s, pl->x, p_cur, pl->partial, cols, history, iterations - 1,
-                     iterations == 1 ? 1 : 0, pl->status, status_out, pl->cfg.image1, pl->cfg.image0, x_out);
+                     dim3(kVecThreads), 0, s, pl->x, p_cur, pl->partial, cols, pl->rho, iterations - 1,
+                     iterations == 1 ? 1 : 0, pl->status, status_out, pl->cfg.image1, pl->cfg.image0, x_out,
+                     history);
   TSN_HIP(hipGetLastError());
   return TSN_OK;
 }
@@ -245,9 +254,17 @@ int check_cg_args(tsn_plan* pl, const void* measurement, uint32_t iterations, fl
                   float lambda, const void* x_out, const void* residuals, const void* status) {
   if (!pl || !measurement || !x_out || !residuals || !status) return fail(TSN_ERR_ARGUMENT, "null argument");
   if (iterations == 0) return fail(TSN_ERR_ARGUMENT, "cg iterations must be positive");
+  if (iterations >= kMaxCgIterations)
+    return fail(TSN_ERR_ARGUMENT, "cg iterations must be below %u", kMaxCgIterations);
   if (!(operator_scale > 0.0f) || !std::isfinite(operator_scale))
     return fail(TSN_ERR_ARGUMENT, "operator scale must be positive and finite");
   if (!(lambda >= 0.0f) || !std::isfinite(lambda)) return fail(TSN_ERR_ARGUMENT, "lambda must be >= 0");
+  // rho_0 .. rho_iterations; a larger buffer replaces the old one, which stays
+  // allocated until the plan is freed (a captured graph may still name it)
+  if (iterations + 1 > pl->rho_cap) {
+    TSN_CHECK(plan_alloc(pl, &pl->rho, iterations + 1));
+    pl->rho_cap = iterations + 1;
+  }
   return TSN_OK;
 }
 
@@ -282,8 +299,10 @@ int tsn_plan_create(const tsn_plan_config* cfg, const tsn_direction* fwd, const 
   const uint32_t grid = c.grid0 * c.grid1;
   if (fwd->h.n_cols != grid || adj->h.output_size != grid || fwd->h.output_size != adj->h.n_cols)
     return fail(TSN_ERR_INCOMPATIBLE, "unexpected operator shape");
-  if (c.precision == TSN_PREC_DENSE16 && (!fwd->tile_dense || !adj->tile_dense))
-    return fail(TSN_ERR_ARGUMENT, "dense control needs dense tiles");
+  if (c.precision == TSN_PREC_DENSE16) {
+    TSN_CHECK(ensure_dense_tiles(fwd));
+    TSN_CHECK(ensure_dense_tiles(adj));
+  }
   if (!fp32) {
     const int want = c.precision == TSN_PREC_SPLITBF16 ? TSN_VALUES_BF16 : TSN_VALUES_FP16;
     if (fwd->values != want || adj->values != want)
@@ -321,6 +340,14 @@ int tsn_plan_create(const tsn_plan_config* cfg, const tsn_direction* fwd, const 
   }
   for (float2** v : {&pl->x, &pl->r, &pl->r2, &pl->p, &pl->p2, &pl->ap, &pl->tmp}) A(v, px);
   A(&pl->partial, 3 * static_cast<size_t>(c.image1));
+  if (!fp32) {
+    A(&pl->fwd_chunks.partials, static_cast<size_t>(fwd->n_slots) * kSlotFloats);
+    A(&pl->fwd_chunks.counters, round_up(fwd->n_counters, 4u));
+    A(&pl->adj_chunks.partials, static_cast<size_t>(adj->n_slots) * kSlotFloats);
+    A(&pl->adj_chunks.counters, round_up(adj->n_counters, 4u));
+  }
+  A(&pl->rho, kRhoInitial);
+  pl->rho_cap = kRhoInitial;
   A(&pl->status, 1);
   if (rc == TSN_OK) {
     const auto t0 = twiddles(c.grid0), t1 = twiddles(c.grid1);
@@ -336,6 +363,10 @@ int tsn_plan_create(const tsn_plan_config* cfg, const tsn_direction* fwd, const 
                        static_cast<const float*>(apodization), c.image0, c.image1, pl->apod_t);
     // the adjoint grid's inactive rows are never written: zero it once here
     rc = zero_async(pl->grid, static_cast<size_t>(grid) * c.coils * sizeof(float2), 0);
+    // arrival counters start at zero (each launch's last arrivals reset them)
+    for (const ChunkScratch* k : {&pl->fwd_chunks, &pl->adj_chunks})
+      if (rc == TSN_OK && k->counters)
+        rc = zero_async(k->counters, round_up(k == &pl->fwd_chunks ? fwd->n_counters : adj->n_counters, 4u) * 4, 0);
     if (rc == TSN_OK) {
       const hipError_t e = hipDeviceSynchronize();
       if (e != hipSuccess) rc = fail(TSN_ERR_RUNTIME, "plan setup: %s", hipGetErrorString(e));
@@ -406,7 +437,8 @@ int tsn_data_consistency(tsn_plan* pl, const void* image, const void* measuremen
     dc.y = reinterpret_cast<const float*>(pl->y_rows);
     dc.row_stride = 2 * coils;
     dc.coil_stride = 2;
-    TSN_CHECK(launch_interp_dc(pl->fwd, interp_kind(pl), pl->pan_il, coils, grid_norm(pl), dc, pl->smp_il, s));
+    TSN_CHECK(launch_interp_dc(pl->fwd, interp_kind(pl), pl->pan_il, coils, grid_norm(pl), dc, pl->smp_il, s,
+                               &pl->fwd_chunks));
   }
   TSN_CHECK(op_adjoint_grid(pl, s));
   TSN_CHECK(op_adjoint_image(pl, pl->ap, nullptr, 0.0f, 1.0f, nullptr, s));

@@ -25,6 +25,10 @@ constexpr int kLongRowEntries = 128;  // FP32 CSR rows longer than this get a wh
 // Device column words of packed tiles hold column * 32 (a byte offset) below
 // bit 31, so a pack may have at most 2^26 columns.
 constexpr uint32_t kMaxDeviceCols = 1u << 26;
+// Per-tile operand offsets are signed 32-bit buffer offsets: sparse operands
+// (1 KiB per tile) hold 2^21 tiles, expanded dense-control tiles (2 KiB) 2^20.
+constexpr uint32_t kMaxDeviceTiles = 1u << 21;
+constexpr uint32_t kMaxDenseTiles = 1u << 20;
 
 void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -44,6 +48,7 @@ int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
   } while (0)
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
+inline uint32_t round_up(uint32_t a, uint32_t b) { return (a + b - 1) / b * b; }
 
 // Block b of a launch runs on XCD b % 8 (the dispatcher's round robin); this
 // gives each XCD a contiguous range of the nb work indices (a permutation of
@@ -171,6 +176,22 @@ void interp_batch_free(InterpBatch* b);
 // 1 row-major float2, 3 interleaved split (FP16 or BF16 as the kind).
 int launch_interp_batch(const InterpBatch& b, int kind, int out_mode, bool conj, float scale, hipStream_t s);
 
+// Scratch of the long-group chunk reduction: partial slots [n_slots][kSlotFloats]
+// and arrival counters [n_counters] (zero between launches; the last arrival
+// of a group resets its counter).  Launches that run at the same time need
+// separate scratch: a direction holds one set for tsn_interp, and every plan
+// holds its own for each of its directions.
+struct ChunkScratch {
+  float* partials = nullptr;
+  uint32_t* counters = nullptr;
+};
+// Packed interpolation (kind and out_mode as below); scratch null: the
+// direction's own.
+int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* res, uint32_t coils,
+                  int out_mode, bool conj, bool accumulate, float scale, float* out_f32,
+                  _Float16* out_hi, _Float16* out_res, hipStream_t stream, bool interleaved,
+                  uint32_t* out_il, const ChunkScratch* scratch = nullptr);
+
 // Data-consistency terms: residual (prediction - scale * y) * w per sample
 // row, y read at row * row_stride + coil * coil_stride + part (floats).
 struct DcTerms {
@@ -181,8 +202,11 @@ struct DcTerms {
 };
 // Forward interpolation of an interleaved split panel with the residual fused
 // into its epilogue; writes the interleaved split residual panel.
+// The dense-control tiles of a packed direction, expanded on the device from
+// its sparse operands on first use (idempotent; not for concurrent callers).
+int ensure_dense_tiles(const tsn_direction* d);
 int launch_interp_dc(const tsn_direction* d, int kind, const uint32_t* panel, uint32_t coils, float scale,
-                     const DcTerms& dc, uint32_t* out_il, hipStream_t s);
+                     const DcTerms& dc, uint32_t* out_il, hipStream_t s, const ChunkScratch* scratch = nullptr);
 // The same residual from stored predictions float2 [rows][coils].
 int launch_dc_residual(const float2* pred, uint32_t rows, uint32_t coils, const DcTerms& dc, bool bf16,
                        uint32_t* out_il, hipStream_t s);
@@ -212,12 +236,13 @@ struct tsn_direction {
   uint32_t* tile_cols = nullptr;     // [tiles * 32] column * 32 | last-tile flag (bit 31, word 2 per 8)
   uint4* tile_a = nullptr;           // [tiles * 64] per lane {re 4xf16, im 4xf16}
   uint16_t* tile_idx = nullptr;      // [tiles * 64] per lane sparse index byte re | im << 8
-  uint4* tile_dense = nullptr;       // [tiles * 64 * 2] per lane {re 8xf16}, {im 8xf16}
+  uint4* tile_dense = nullptr;       // [tiles * 64 * 2] per lane {re 8xf16}, {im 8xf16}; built on first use
   tsn::WorkItem* items = nullptr;    // runs of groups with <= kSplitAboveTiles tiles each
   tsn::ChunkItem* chunks = nullptr;  // chunks of the longer groups (fixed by the pack)
   float* partials = nullptr;         // [slots][kSlotFloats] partial sums of the chunks
   uint32_t* counters = nullptr;      // [long groups] arrival counters, zero between launches
   uint32_t n_items = 0, n_chunks = 0;
+  uint32_t n_slots = 0, n_counters = 0;  // sizes of the chunk scratch (ChunkScratch)
   uint32_t unit_tiles = 0;           // tile budget of one regular unit
   std::vector<uint32_t> host_offsets;  // group_offsets kept for rescheduling
   uint32_t* order = nullptr;         // [n_items + n_chunks] launch order (kChunkTag marks chunks)

@@ -52,7 +52,7 @@ class Plan:
         check(lib().tsn_plan_create(C.byref(cfg), forward.handle, adjoint.handle, _ptr(smaps), _ptr(apod), C.byref(out)))
         self._h = out
         self._dirs = (forward, adjoint)  # keep the directions alive
-        self._graph_key = None
+        self._keep_call = None  # tensors a captured graph reads and writes
 
     def __del__(self):
         # At interpreter exit the process is ending and module globals may
@@ -173,7 +173,9 @@ class Plan:
             raise ValueError("graph capture needs a non-default stream (use torch.cuda.stream(...))")
         check(lib().tsn_cg_graph_capture(self._h, _ptr(measurement), iterations, float(operator_scale), float(lam),
                                          _ptr(out.image), _ptr(out.residuals), _ptr(out.status), _stream()))
-        self._graph_key = (measurement.data_ptr(), out.image.data_ptr())
+        # the graph holds raw device addresses: keep its buffers alive until
+        # the next capture, so the caching allocator cannot hand them out
+        self._keep_call = (measurement, out)
         return out
 
     def replay(self) -> None:
