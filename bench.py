@@ -263,37 +263,26 @@ def time_launches(torch, fn, steps, warmup):
     return start.elapsed_time(stop) / steps  # ms
 
 
-TRAFFIC_FILES = {"fetch": "profiles/interp_il_256x512_8coil_radial402_fetch_size.csv",
-                 "write": "profiles/interp_il_256x512_8coil_radial402_write_size.csv"}
+TRAFFIC_TABLE = "profiles/hbm_traffic_256x512_8coil_radial402.md"
 COPY_BYTES = 64 << 20  # calibration copy of --roofline-only
 
 
 def measured_traffic(kernel_substr: str):
-    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
-    FETCH_SIZE and WRITE_SIZE runs of ``bench.py --roofline-only`` (KiB per
-    dispatch), each counter scaled by what it reads for the 64 MiB
-    device-to-device copy of the same run.  None if the files are absent."""
+    """HBM bytes per launch of a kernel from the committed summary of the
+    rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of ``bench.py
+    --roofline-only``: the table's last column, KiB per dispatch scaled by what
+    each counter reads for the run's 64 MiB device-to-device copy.  None if the
+    table or the kernel's row is absent."""
 
-    import csv
-    root = os.path.dirname(os.path.abspath(__file__))
-    total = 0.0
-    for counter, rel in TRAFFIC_FILES.items():
-        path = os.path.join(root, rel)
-        if not os.path.exists(path):
-            return None
-        kern, copy = [], []
-        with open(path, newline="") as f:
-            for row in csv.DictReader(f):
-                v = float(row["Counter_Value"])
-                if kernel_substr in row["Kernel_Name"]:
-                    kern.append(v)
-                elif "copyBuffer" in row["Kernel_Name"] and v > 1024:
-                    copy.append(v)
-        if not kern or not copy:
-            return None
-        factor = COPY_BYTES / (1024.0 * sum(copy) / len(copy))
-        total += 1024.0 * sum(kern) / len(kern) * factor
-    return total
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_TABLE)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 7 and cells[0].strip("`") == kernel_substr:
+                return float(cells[6])
+    return None
 
 
 def interp_roofline(torch, packs, steps, warmup, log):
@@ -333,7 +322,7 @@ def interp_roofline(torch, packs, steps, warmup, log):
             f"HBM traffic {res[name]['traffic']})")
     res["dominant"] = res["adjoint"] if adj_ms >= fwd_ms else res["forward"]
     res["traffic_source"] = (f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --roofline-only, "
-                             f"scaled by a {COPY_BYTES >> 20} MiB copy: {', '.join(TRAFFIC_FILES.values())}")
+                             f"scaled by a {COPY_BYTES >> 20} MiB copy: {TRAFFIC_TABLE}")
     return res
 
 

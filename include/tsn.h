@@ -118,7 +118,7 @@ int tsn_kb_arrays(const tsn_kb_matrix* kb, uint32_t* rows, uint32_t* width, cons
 int tsn_kb_apodization(uint32_t image0, uint32_t image1, uint32_t grid0, uint32_t grid1,
                        uint32_t numpoints, float* out);
 /* Pack view 0 (G, rows = samples) or 1 (G_T, rows = active grid points)
- * with the production planner (policy 0 = morton, 1 = trajectory order),
+ * with the production planner (policy 0 = morton, 1 = trajectory order, 2 = overlap),
  * the reference default seed being 20260823. */
 int tsn_kb_pack(const tsn_kb_matrix* kb, int view, int policy, uint64_t seed, int threads,
                 tsn_host_pack** out);

@@ -76,3 +76,41 @@ def test_reference_pack_reproduces_reference_bin(layout_golden, case, dense, tmp
     got = _run(d_real, x, dense)
     want = ref[:, :8].T + 1j * ref[:, 8:].T
     _gate(got, want)
+
+
+PRODUCTION_FILES = ("header", "group_offsets", "group_row_ids", "tile_col_ids", "tile_a_comp", "tile_pair_meta",
+                    "dense_input", "reference")
+
+
+@pytest.mark.parametrize("dense", [False, True], ids=["smfmac", "dense_mfma"])
+@pytest.mark.parametrize("view", ["G", "G_T"])
+def test_production_export_reproduces_its_reference_bin(view, dense, tmp_path):
+    """The reference's production exporter (``src/export_torchkbnufft_gpu_case.py``
+    ``export_component`` :377-489) writes a real and an imaginary directory,
+    each with dense_input.bin and its own reference.bin (that component's FP16
+    coefficients x dense_input).  Loaded as one complex direction:
+    Re y_c = ref_re[:, c] - ref_im[:, 8 + c], Im y_c = ref_re[:, 8 + c] + ref_im[:, c]."""
+
+    import os
+    from trajsparsenufft_amd.device import Direction
+    path = os.path.join(os.path.dirname(__file__), "golden", "production_golden.npz")
+    with np.load(path, allow_pickle=False) as data:
+        g = {k: data[k] for k in data.files}
+    dirs = {}
+    for comp in ("real", "imag"):
+        d = tmp_path / comp
+        d.mkdir()
+        for name in PRODUCTION_FILES:
+            (d / f"{name}.bin").write_bytes(g[f"{view}_morton_{comp}_{name}"].tobytes())
+        dirs[comp] = d
+    print(f"[refpack] production view={view} dense={dense}", flush=True)
+    direction = Direction.load(dirs["real"], dirs["imag"])
+    x = np.frombuffer(g[f"{view}_morton_real_dense_input"].tobytes(), dtype=np.float16)
+    x_im = np.frombuffer(g[f"{view}_morton_imag_dense_input"].tobytes(), dtype=np.float16)
+    assert np.array_equal(x.view(np.uint16), x_im.view(np.uint16))
+    ref_re = np.frombuffer(g[f"{view}_morton_real_reference"].tobytes(), dtype=np.float32).reshape(-1, 16)
+    ref_im = np.frombuffer(g[f"{view}_morton_imag_reference"].tobytes(), dtype=np.float32).reshape(-1, 16)
+    ref_re, ref_im = ref_re.astype(np.float64), ref_im.astype(np.float64)
+    got = _run(direction, x, dense)
+    want = (ref_re[:, :8] - ref_im[:, 8:]).T + 1j * (ref_re[:, 8:] + ref_im[:, :8]).T
+    _gate(got, want)

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <iterator>
 #include <numeric>
 #include <thread>
 
@@ -514,12 +515,72 @@ void compress(const View& v, const std::vector<float>& vals, const int32_t* loca
   }
 }
 
+// Size of the intersection of two ascending column lists.
+uint32_t common_count(const int32_t* a, uint32_t na, const std::vector<int32_t>& b) {
+  uint32_t i = 0, j = 0, n = 0;
+  while (i < na && j < b.size()) {
+    if (a[i] < b[j]) ++i;
+    else if (b[j] < a[i]) ++j;
+    else ++n, ++i, ++j;
+  }
+  return n;
+}
+
+// Row grouping "overlap" (src/nufft_layout_oracle.py:243-309): within windows
+// of 4 groups of the Morton order, each group starts with the window's first
+// remaining row and then repeatedly takes the remaining row sharing the most
+// columns with the group's union, then the fewest new columns, then the
+// earliest in the window.
+std::vector<int32_t> overlap_order(const View& v, const std::vector<int32_t>& morton) {
+  const uint32_t window = 4 * kM;
+  std::vector<int32_t> ordered;
+  ordered.reserve(morton.size());
+  for (size_t start = 0; start < morton.size(); start += window) {
+    std::vector<int32_t> pool(morton.begin() + start, morton.begin() + std::min(morton.size(), start + window));
+    while (!pool.empty()) {
+      std::vector<int32_t> group{pool.front()};
+      pool.erase(pool.begin());
+      const int32_t r0 = group[0];
+      std::vector<int32_t> uni(v.cols.begin() + v.ptr[r0], v.cols.begin() + v.ptr[r0 + 1]);
+      std::sort(uni.begin(), uni.end());
+      uni.erase(std::unique(uni.begin(), uni.end()), uni.end());
+      while (!pool.empty() && group.size() < static_cast<size_t>(kM)) {
+        size_t pick = 0;
+        int64_t best_common = -1, best_new = 0;
+        for (size_t i = 0; i < pool.size(); ++i) {
+          const int32_t r = pool[i];
+          const uint32_t len = v.ptr[r + 1] - v.ptr[r];
+          const int64_t common = common_count(v.cols.data() + v.ptr[r], len, uni);
+          const int64_t fresh = static_cast<int64_t>(len) - common;
+          // strictly better only: ties keep the earlier row (the key's last term)
+          if (common > best_common || (common == best_common && fresh < best_new)) {
+            pick = i;
+            best_common = common;
+            best_new = fresh;
+          }
+        }
+        const int32_t chosen = pool[pick];
+        pool.erase(pool.begin() + static_cast<std::ptrdiff_t>(pick));
+        group.push_back(chosen);
+        std::vector<int32_t> merged;
+        merged.reserve(uni.size() + (v.ptr[chosen + 1] - v.ptr[chosen]));
+        std::set_union(uni.begin(), uni.end(), v.cols.begin() + v.ptr[chosen], v.cols.begin() + v.ptr[chosen + 1],
+                       std::back_inserter(merged));
+        uni.swap(merged);
+      }
+      ordered.insert(ordered.end(), group.begin(), group.end());
+    }
+  }
+  return ordered;
+}
+
 int pack_view(const View& v, int policy, uint64_t seed, int threads, bool bf16, HostPack& out) {
   const uint32_t n = static_cast<uint32_t>(v.row_ids.size());
   std::vector<int32_t> order(n);
   std::iota(order.begin(), order.end(), 0);
-  if (policy == 0)
+  if (policy == 0 || policy == 2)
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return v.morton[a] < v.morton[b]; });
+  if (policy == 2) order = overlap_order(v, order);
   const uint32_t groups = (n + kM - 1) / kM;
   std::vector<int32_t> local(static_cast<size_t>(groups) * kM, -1);
   for (uint32_t i = 0; i < n; ++i) local[i] = order[i];
@@ -677,7 +738,7 @@ int tsn_kb_apodization(uint32_t image0, uint32_t image1, uint32_t grid0, uint32_
 
 int tsn_kb_pack_values(const tsn_kb_matrix* kb, int view, int policy, uint64_t seed, int threads, int values,
                        tsn_host_pack** out) {
-  if (!kb || !out || (view != 0 && view != 1) || (policy != 0 && policy != 1) ||
+  if (!kb || !out || (view != 0 && view != 1) || policy < 0 || policy > 2 ||
       (values != TSN_VALUES_FP16 && values != TSN_VALUES_BF16))
     return fail(TSN_ERR_ARGUMENT, "bad arguments");
   *out = nullptr;

@@ -27,7 +27,7 @@ from ._lib import PackHeader, check, lib
 
 DEFAULT_SEED = 20260823
 VIEWS = {"G": 0, "G_T": 1}
-POLICIES = {"morton": 0, "trajectory": 1}
+POLICIES = {"morton": 0, "trajectory": 1, "overlap": 2}
 HEADER_FIELDS = [name for name, _ in PackHeader._fields_]
 
 

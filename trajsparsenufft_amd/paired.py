@@ -56,15 +56,10 @@ def sha256(path: Path) -> str:
 
 
 def quantile(values: Sequence[float], probability: float) -> float:
-    """Linear-interpolation quantile of a sample."""
+    """Sample quantile by linear interpolation between order statistics
+    (numpy's default rule: position probability * (n - 1))."""
 
-    ordered = sorted(values)
-    position = probability * (len(ordered) - 1)
-    low, high = math.floor(position), math.ceil(position)
-    if low == high:
-        return ordered[low]
-    weight = position - low
-    return ordered[low] * (1.0 - weight) + ordered[high] * weight
+    return float(np.quantile(np.asarray(values, dtype=np.float64), probability))
 
 
 def bootstrap_geomean_ci(ratios: Sequence[float], seed: int, draws: int = BOOTSTRAP_DRAWS) -> dict:
@@ -81,14 +76,14 @@ def bootstrap_geomean_ci(ratios: Sequence[float], seed: int, draws: int = BOOTST
 
 
 def parse_result(stdout: str) -> dict:
-    """The first JSON line of a process's output that reports a correct run with a time."""
+    """The result object of a run: the first line of output that parses as a
+    JSON object with ``correct`` true and a time ``us``."""
 
-    for line in stdout.splitlines():
-        if line.startswith("{"):
-            payload = json.loads(line)
-            if payload.get("correct") and "us" in payload:
-                return payload
-    raise RuntimeError("missing benchmark result")
+    candidates = (json.loads(line) for line in stdout.splitlines() if line.lstrip().startswith("{"))
+    found = next((obj for obj in candidates if obj.get("correct") is True and "us" in obj), None)
+    if found is None:
+        raise RuntimeError("no correct timed result line in the output (missing)")
+    return found
 
 
 # ------------------------------------------------------------ GPU audit
@@ -153,7 +148,7 @@ def run_paired(variants: dict[str, Command], cases: Sequence[str], output: Path,
     name_a, name_b = list(variants)
     output = Path(output)
     if output.exists() and any(output.iterdir()):
-        raise FileExistsError(f"refusing to overwrite nonempty {output}")
+        raise FileExistsError(f"output directory {output} already holds files; give an empty or new one")
     raw = output / "raw_processes"
     raw.mkdir(parents=True, exist_ok=True)
     records, snapshots, paired = [], [], []
@@ -290,16 +285,16 @@ def validate_rows(rows: list[dict], cases: Sequence[str], frames: int, iteration
     expected = set(itertools.product(cases, range(frames)))
     keys = [(r["case"], r["frame"]) for r in rows]
     if len(keys) != len(expected) or set(keys) != expected:
-        raise RuntimeError("quality matrix has missing, duplicate, or unexpected rows")
+        raise RuntimeError("quality rows do not cover each (case, frame) exactly once: some are missing or repeated")
     for row in rows:
         if row["cg_iterations"] != iterations:
             raise RuntimeError(f"quality row does not use {iterations} CG steps")
         if not all(row[k] is True for k in ("finite", "application_vs_fp32_pass", "eager_graph_reconstruction_bitwise",
                                             "eager_graph_residuals_bitwise")):
-            raise RuntimeError("row-level quality or eager/Graph determinism failed")
+            raise RuntimeError(f"row {row['case']}/{row['frame']} fails its image gates or eager/Graph determinism")
         metrics = [float(row[k]) for k in ("native_vs_fp32_magnitude_nrmse", "native_vs_fp32_ssim")]
         if not all(math.isfinite(v) for v in metrics):
-            raise RuntimeError("nonfinite reconstruction metric")
+            raise RuntimeError(f"row {row['case']}/{row['frame']} has a metric that is not finite")
 
 
 # ------------------------------------------------------------ CLI
