@@ -21,9 +21,10 @@ hi + residual, one direction) are launched on their own and timed with events
 on the launching stream; algorithmic bytes per launch
 B = nnz*8 + 2*n_in*2C*2 + n_out*C*8 (nnz = 36*M).
 
-CPU baseline: the oracle's FP32 CSR + FFT CG-SENSE (oracle/cpu_recon.py) on one
-host thread, one full reconstruction (or a bounded number of CG iterations,
-extrapolated), on rank 0 only.
+CPU baseline: the oracle's FP32 CSR + FFT CG-SENSE (oracle/cpu_recon.py, a
+multi-core C CSR product) on the host cores, one full reconstruction (or a
+bounded number of CG iterations, extrapolated), and at 128^2 / 4 coils the same
+FP32 CG-SENSE beside the FP64 direct-NUDFT CG-SENSE.
 """
 
 from __future__ import annotations
@@ -56,6 +57,7 @@ def parse_args():
     p.add_argument("--no-batches", action="store_true",
                    help="skip the batched workloads (48 uneven slices; 24 frames at 768^2 / 16 coils)")
     p.add_argument("--batches-only", action="store_true", help="only the batched workloads")
+    p.add_argument("--no-sweep", action="store_true", help="skip the spiral precision sweep")
     return p.parse_args()
 
 
@@ -89,30 +91,17 @@ def batch_workloads(torch, log, reps=3):
     Data come from the FP32 operator of each slice plus complex Gaussian
     noise (the FP64 NUDFT of 48 + 24 images would dominate the run)."""
 
-    from trajsparsenufft_amd import device, pack, synth
+    from trajsparsenufft_amd import device, workloads
     from trajsparsenufft_amd.device import Direction
     from trajsparsenufft_amd.recon import build_batch, build_plan
     out = {}
     # ---------------- slices
-    rng = np.random.default_rng(20260903)
     n_slices = 48
-    spokes = rng.integers(64, 403, n_slices)
-    coils = rng.choice([4, 8, 12, 16], n_slices)
     t0 = time.perf_counter()
-    cases, meas, scales, plans = [], [], [], []
-    for i in range(n_slices):
-        traj = synth.radial(int(spokes[i]), READOUT, golden=True, offset=float(rng.uniform(0, np.pi)))
-        packs = pack.prepare_trajectory(traj, (IMAGE, IMAGE), (GRID, GRID))
-        maps = torch.from_numpy(synth.coil_maps(int(coils[i]), IMAGE)).cuda()
-        x = torch.from_numpy(synth.phantom(IMAGE, seed=100 + i)).cuda()
-        # the plan's forward is normalised by 1/sqrt(K0 K1); measurements are raw
-        y = build_plan(packs, maps, "fp32").forward(x).cpu().numpy() * GRID
-        y = torch.from_numpy(synth.add_noise(y, SNR_DB, seed=200 + i)).cuda()
-        plan = build_plan(packs, maps, "split16", autotune=True)
-        cases.append((packs, maps))
-        meas.append(y)
-        scales.append(plan.operator_scale(y))
-        plans.append(plan)
+    w = workloads.uneven_slices(n_slices)
+    cases, meas, scales = w.cases, w.measurements, w.scales
+    spokes, coils = w.spokes, w.coils
+    plans = [build_plan(packs, maps, "split16", autotune=True) for packs, maps in cases]
     prep = time.perf_counter() - t0
     batch = build_batch(cases, "split16", autotune=True)
     stream = torch.cuda.Stream()
@@ -133,28 +122,23 @@ def batch_workloads(torch, log, reps=3):
                      "batched_recon_per_s": n_slices * 1000.0 / batch_ms, "batched_ms": batch_ms,
                      "loop_recon_per_s": n_slices * 1000.0 / loop_ms, "loop_ms": loop_ms,
                      "speedup_vs_loop": loop_ms / batch_ms, "bitwise_equal_to_per_slice": same,
-                     "spokes": [int(v) for v in spokes], "coils": [int(v) for v in coils], "prep_seconds": prep}
+                     "spokes": [int(v) for v in spokes], "coils": [int(v) for v in coils],
+                     "batch_tuning": batch.tuning, "prep_seconds": prep}
     log(f"[bench] slices: batched {batch_ms:.2f} ms ({n_slices * 1000.0 / batch_ms:.0f} recon/s), per-slice loop "
         f"{loop_ms:.2f} ms, bitwise {same}")
-    del batch, outs, singles, plans, cases, meas
+    del batch, outs, singles, plans, cases, meas, w
     torch.cuda.empty_cache()
     # ---------------- frames
-    n0, grid, nc, frames, iters = 384, 768, 16, 24, 30
+    nc, frames, iters = workloads.FRAME_COILS, 24, 30
     t0 = time.perf_counter()
-    traj = synth.radial(600, 768, golden=True)
-    packs = pack.prepare_trajectory(traj, (n0, n0), (grid, grid))
-    maps = torch.from_numpy(synth.coil_maps(nc, n0)).cuda()
+    w = workloads.dynamic_frames(frames)
+    packs, maps = w.cases[0]
+    meas, truth, scale = w.measurements, w.truth, w.scales[0]
     fp32 = build_plan(packs, maps, "fp32")
-    truth = [synth.phantom(n0, n0, seed=300, motion=0.05, frame=f, frames=frames) for f in range(frames)]
-    meas = []
-    for f, x in enumerate(truth):
-        y = fp32.forward(torch.from_numpy(x).cuda()).cpu().numpy() * grid  # raw measurement scale
-        meas.append(torch.from_numpy(synth.add_noise(y, 35.0, seed=400 + f)).cuda())
-    scale = fp32.operator_scale(meas[0])
     prep = time.perf_counter() - t0
-    batch = build_batch([(packs, maps)] * frames, "split16", shared=True, autotune=True)
+    batch = build_batch(w.cases, "split16", shared=True, autotune=True)
     with torch.cuda.stream(stream):
-        outs = batch.capture_cg(meas, iters, [scale] * frames, LAMBDA)
+        outs = batch.capture_cg(meas, iters, w.scales, LAMBDA)
         frames_ms = time_once(torch, batch.replay, reps)
         stream.synchronize()
     nrmse, truth_err = [], []
@@ -178,22 +162,81 @@ def batch_workloads(torch, log, reps=3):
     gp, sp = panel(g), panel(m)
     out_s = torch.empty((2, m, 16), dtype=torch.int32, device="cuda")
     out_g = torch.zeros((g, nc), dtype=torch.complex64, device="cuda")
-    f_ms = time_launches(torch, lambda: device.interp_il(fwd, gp, out_s, nc, False, False, 1.0, "split_il"), 20, 5)
-    a_ms = time_launches(torch, lambda: device.interp_il(adj, sp, out_g, nc, True, False, 1.0, "row_major"), 20, 5)
+    tuned = {"forward": fwd.coil_waves, "adjoint": adj.coil_waves}
     interp = {}
-    for name, ms, n_in, n_out in (("forward", f_ms, g, m), ("adjoint", a_ms, m, g)):
-        b = nnz * 8 + 2 * n_in * 2 * nc * 2 + n_out * nc * 8
-        interp[name] = {"us": ms * 1e3, "bytes": b, "GBs": b / (ms * 1e-3) / 1e9,
-                        "frac_of_peak": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    for waves in (1, 2):
+        fwd.coil_waves = adj.coil_waves = waves
+        f_ms = time_launches(torch, lambda: device.interp_il(fwd, gp, out_s, nc, False, False, 1.0, "split_il"), 20, 5)
+        a_ms = time_launches(torch, lambda: device.interp_il(adj, sp, out_g, nc, True, False, 1.0, "row_major"), 20, 5)
+        for name, ms, n_in, n_out in (("forward", f_ms, g, m), ("adjoint", a_ms, m, g)):
+            b = nnz * 8 + 2 * n_in * 2 * nc * 2 + n_out * nc * 8
+            row = {"us": ms * 1e3, "bytes": b, "GBs": b / (ms * 1e-3) / 1e9,
+                   "frac_of_peak": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            interp.setdefault(name, {})[f"coil_waves_{waves}"] = row
+            if waves == tuned[name]:
+                interp[name].update(row)
+        log(f"[bench] frames interp, {waves} wave(s) per entry: forward {f_ms * 1e3:.1f} us, adjoint {a_ms * 1e3:.1f} us")
+    interp["tuned_coil_waves"] = tuned
     out["frames"] = {"workload": "24 frames 384x384 / 768x768, 16 coils, shared golden-angle radial 600x768 pack, "
                                  "35 dB, 30 CG batched across frames, split16",
                      "frames_per_s": frames * 1000.0 / frames_ms, "batched_ms": frames_ms,
                      "nrmse_vs_fp32": {"max": max(nrmse), "mean": float(np.mean(nrmse))},
                      "nrmse_vs_truth": {"max": max(truth_err), "mean": float(np.mean(truth_err))},
-                     "interp": interp, "prep_seconds": prep}
+                     "interp": interp, "batch_tuning": batch.tuning, "prep_seconds": prep}
     log(f"[bench] frames: {frames_ms:.1f} ms for {frames} frames ({frames * 1000.0 / frames_ms:.1f} frames/s), "
         f"NRMSE vs FP32 max {max(nrmse):.2e}, interp fwd {interp['forward']['GBs']:.0f} / adj "
         f"{interp['adjoint']['GBs']:.0f} GB/s")
+    return out
+
+
+def spiral_sweep(torch, variants, log):
+    """Precision sweep at the headline shape with a variable-density spiral
+    (16 interleaves x 8192 samples, density ~ r^-1.5; 256^2 / 512^2, 8 coils,
+    40 dB, 10 CG): per variant the magnitude NRMSE against the FP64
+    direct-NUDFT CG, the change of the error against the truth relative to the
+    FP64 CG's, the adjointness defect |<Ax, y> - <x, A^H y>| / max of both,
+    and the graph-replay latency."""
+
+    from trajsparsenufft_amd import pack, synth
+    from trajsparsenufft_amd.recon import build_plan, check_cg
+    traj = synth.spiral(16, 8192)
+    x = synth.phantom(IMAGE, seed=4)
+    smaps = synth.coil_maps(COILS, IMAGE)
+    ref = Fp64Nudft(torch, traj, smaps, IMAGE)
+    y = ref.forward(torch.from_numpy(x).to(torch.complex128).cuda()).cpu().numpy()
+    y = synth.add_noise(y, SNR_DB, seed=5)
+    x64 = ref.cg(torch.from_numpy(y).to(torch.complex128).cuda(), CG_ITERS, LAMBDA, GRID).cpu().numpy()
+    del ref
+    packs = pack.prepare_trajectory(traj, (IMAGE, IMAGE), (GRID, GRID))
+    smaps_dev = torch.from_numpy(smaps).cuda()
+    y_dev = torch.from_numpy(y).cuda()
+    rng = np.random.default_rng(6)
+    xs = (rng.standard_normal((IMAGE, IMAGE)) + 1j * rng.standard_normal((IMAGE, IMAGE))).astype(np.complex64)
+    ys = (rng.standard_normal(y.shape) + 1j * rng.standard_normal(y.shape)).astype(np.complex64)
+    fp64_truth = magnitude_nrmse(x64, x)
+    out = {"workload": "spiral 16 interleaves x 8192 (density r^-1.5), 256x256 / 512x512, 8 coils, 40 dB, 10 CG",
+           "fp64_vs_truth": fp64_truth, "variants": {}}
+    for v in variants:
+        plan = build_plan(packs, smaps_dev, v)
+        scale = plan.operator_scale(y_dev)
+        res = plan.cg(y_dev, CG_ITERS, scale, LAMBDA)
+        torch.cuda.synchronize()
+        check_cg(res)
+        img = res.image.cpu().numpy()
+        left = np.vdot(plan.forward(torch.from_numpy(xs).cuda()).cpu().numpy().astype(np.complex128), ys)
+        right = np.vdot(xs.astype(np.complex128), plan.adjoint(torch.from_numpy(ys).cuda()).cpu().numpy())
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            plan.capture_cg(y_dev, CG_ITERS, scale, LAMBDA)
+            graph_ms = time_launches(torch, plan.replay, 10, 3)
+            stream.synchronize()
+        row = {"nrmse_vs_fp64": magnitude_nrmse(img, x64),
+               "truth_delta": magnitude_nrmse(img, x) - fp64_truth,
+               "adjointness": float(abs(left - right) / max(abs(left), abs(right))),
+               "graph_ms": graph_ms}
+        out["variants"][v] = row
+        log(f"[bench] sweep {v}: NRMSE vs FP64 CG {row['nrmse_vs_fp64']:.2e}, truth delta {row['truth_delta']:+.2e}, "
+            f"adjointness {row['adjointness']:.1e}, {graph_ms:.3f} ms")
     return out
 
 
@@ -353,6 +396,84 @@ def roofline_only(args) -> int:
     return 0
 
 
+def host_cores() -> int:
+    """Threads of the CPU baseline: the cores in this process's affinity,
+    capped by OMP_NUM_THREADS when the host sets it (the host's share)."""
+
+    cores = len(os.sched_getaffinity(0))
+    try:
+        cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    except ValueError:
+        pass
+    return max(1, cores)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(packs, smaps, y, scale, budget_s, log):
+    """The oracle's CPU CG-SENSE, timed on the host cores (oracle/cpu_recon.py
+    with the multi-core C CSR product, scipy.fft on the same threads):
+
+    * FP32 CSR + FFT CG-SENSE at the benchmark shape (256^2 / 512^2, 8 coils,
+      402 x 512 radial, 10 CG + the right-hand side): ``value``; if one
+      reconstruction would exceed the budget, a bounded number of CG
+      iterations is timed and scaled to the full solve;
+    * at 128^2 / 256^2, 4 coils, 96 x 256 radial, 10 CG: the same FP32 CSR
+      CG-SENSE and the FP64 direct-NUDFT CG-SENSE (BLAS on the same threads)."""
+
+    import threadpoolctl
+    from oracle import nudft
+    from oracle.cpu_recon import CpuSense
+    from trajsparsenufft_amd import pack, synth
+    cores = host_cores()
+    model = cpu_model()
+    cols, re_, im_ = packs.kb.arrays()
+    with threadpoolctl.threadpool_limits(cores):
+        op = CpuSense(cols, re_, im_, smaps, packs.apod, (IMAGE, IMAGE), (GRID, GRID), workers=cores, backend="omp")
+        t_apply = time.perf_counter()
+        op.adjoint(op.forward(synth.phantom(IMAGE, seed=0)))
+        apply_s = time.perf_counter() - t_apply
+        applies_per_recon = 1 + 2 * CG_ITERS  # RHS adjoint + (forward + adjoint) per iteration
+        iters = CG_ITERS if apply_s * 2 * (CG_ITERS + 1) <= budget_s else max(1, int(budget_s / (2 * apply_s)) - 1)
+        t_cpu = time.perf_counter()
+        op.cg(y, iters, scale)
+        cpu_s = time.perf_counter() - t_cpu
+        per_recon = cpu_s * applies_per_recon / (1 + 2 * iters)
+        # the 128^2 plain configuration: FP32 CSR and the FP64 NUDFT oracle
+        traj0 = synth.radial(96, 256, golden=True)
+        packs0 = pack.prepare_trajectory(traj0, (128, 128), (256, 256))
+        x0, maps0 = synth.phantom(128, seed=0), synth.coil_maps(4, 128)
+        omega0 = (traj0.T * 2.0 * np.pi).astype(np.float32)
+        y0 = synth.add_noise(nudft.nudft_forward(x0, maps0, omega0), SNR_DB, seed=1)
+        c0, r0, i0 = packs0.kb.arrays()
+        op0 = CpuSense(c0, r0, i0, maps0, packs0.apod, (128, 128), (256, 256), workers=cores, backend="omp")
+        s0 = float(np.sqrt(np.abs(op0.adjoint(y0 / 256.0)).max()))
+        t0 = time.perf_counter()
+        x32, _, _ = op0.cg(y0, CG_ITERS, s0)
+        fp32_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        x64, _, _ = nudft.nudft_cg_sense(y0, maps0, omega0, (256, 256), CG_ITERS, LAMBDA)
+        fp64_s = time.perf_counter() - t0
+    log(f"[bench] cpu baseline ({cores} threads, {model}): {per_recon:.2f} s per 256^2 reconstruction; "
+        f"128^2: FP32 CSR {fp32_s:.2f} s, FP64 NUDFT {fp64_s:.2f} s")
+    return {"value": 1.0 / per_recon, "unit": "recon/s", "cores": cores, "kind": "port",
+            "sample": f"oracle FP32 CSR (C, OpenMP) + scipy.fft CG-SENSE at the benchmark shape on {cores} threads, "
+                      f"{iters} CG iterations timed ({cpu_s:.1f} s) and scaled to 10 iterations + RHS",
+            "cpu_model": model, "seconds_per_recon": per_recon,
+            "plain_128": {"workload": "128x128 / 256x256 / 4 coils / golden-angle radial 96x256 / 10 CG",
+                          "fp32_csr_seconds": fp32_s, "fp64_nudft_seconds": fp64_s,
+                          "fp32_vs_fp64_nrmse": magnitude_nrmse(x32, x64)}}
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -427,28 +548,10 @@ def main() -> int:
     roof = interp_roofline(torch, packs, max(args.steps, 20), args.warmup, log)
     dominant = roof["dominant"]
 
-    # ---- CPU baseline (oracle restatement, one host thread)
+    # ---- CPU baseline (the oracle's CPU reconstructions on the host cores)
     cpu = None
     if not args.no_cpu_baseline:
-        from oracle.cpu_recon import CpuSense
-        cols, re_, im_ = packs.kb.arrays()
-        op = CpuSense(cols, re_, im_, smaps, packs.apod, (IMAGE, IMAGE), (GRID, GRID), workers=1)
-        t_apply = time.perf_counter()
-        op.adjoint(op.forward(images["split16"] if "split16" in images else x_true))
-        apply_s = time.perf_counter() - t_apply
-        applies_per_recon = 1 + 2 * CG_ITERS  # RHS adjoint + (forward + adjoint) per iteration
-        iters = CG_ITERS if apply_s * 2 * (CG_ITERS + 1) <= args.cpu_budget_s else \
-            max(1, int(args.cpu_budget_s / (2 * apply_s)) - 1)
-        t_cpu = time.perf_counter()
-        op.cg(y, iters, scale)
-        cpu_s = time.perf_counter() - t_cpu
-        per_recon = cpu_s * applies_per_recon / (1 + 2 * iters)
-        cpu = {"value": 1.0 / per_recon, "unit": "recon/s", "cores": 1, "kind": "port",
-               "sample": f"oracle FP32 CSR + scipy.fft CG-SENSE at the same shape, {iters} CG iterations timed "
-                         f"({cpu_s:.1f} s), scaled to 10 iterations + RHS; host {platform.processor() or platform.machine()}",
-               "seconds_per_recon": per_recon}
-        log(f"[bench] cpu baseline {per_recon:.2f} s per reconstruction ({iters} iterations timed)")
-
+        cpu = cpu_baseline(packs, smaps, y, scale, args.cpu_budget_s, log)
     head = timings["split16"]
     value = 1000.0 / head["graph_ms"]
     out = {
@@ -482,8 +585,11 @@ def main() -> int:
         "prep_seconds": prep_s,
         "cpu_baseline": cpu,
     }
-    if not args.no_batches:
+    if not args.no_sweep:
         del plans
+        torch.cuda.empty_cache()
+        out["sweep"] = spiral_sweep(torch, variants, log)
+    if not args.no_batches:
         torch.cuda.empty_cache()
         out["batches"] = batch_workloads(torch, log)
     print(json.dumps(out), flush=True)

@@ -149,11 +149,20 @@ int tsn_direction_header(const tsn_direction* dir, tsn_pack_header* out);
 int tsn_direction_schedule(tsn_direction* dir, uint32_t unit_tiles);
 /* Tile budget of the current schedule (0 for CSR directions). */
 uint32_t tsn_direction_unit_tiles(const tsn_direction* dir);
+/* Waves per work entry of interleaved-panel interpolation of 9..16 coils
+ * (tsn_interp_il, plans, batches): 1 = one wave computes both coil blocks
+ * from the shared tile operands (the default, measured faster at 768^2 / 16
+ * coils), 2 = one wave per coil block on the same entry.  Results are
+ * bitwise the same.  No counterpart in the reference (8 coils, one warp per
+ * group). */
+int tsn_direction_set_coil_waves(tsn_direction* dir, uint32_t waves);
+uint32_t tsn_direction_coil_waves(const tsn_direction* dir);
 /* Autotuner: time the interpolation of this direction for `coils` coils over
- * the candidate unit sizes {2,3,4,5,6,8,10,12,14,16} (scratch panels allocated
- * and freed here, the stream synchronized) and keep the fastest schedule.
- * Results do not depend on the schedule (bitwise).  No counterpart in the
- * reference; run once per trajectory and coil count. */
+ * the candidate unit sizes {2,3,4,5,6,8,10,12,14,16}, and for 9..16 coils
+ * over both coil-wave settings (scratch panels allocated and freed here, the
+ * stream synchronized), and keep the fastest schedule.  Results do not depend
+ * on the schedule (bitwise).  No counterpart in the reference; run once per
+ * trajectory and coil count. */
 int tsn_direction_autotune(tsn_direction* dir, uint32_t coils, void* stream, uint32_t* best_unit_tiles,
                            float* best_us);
 /* Device bytes the direction occupies (pack + index arrays, plus the
@@ -310,6 +319,15 @@ size_t tsn_batch_bytes(const tsn_batch* batch);
  * x_out[z] float2 [image0][image1], residuals_out[z] double[iterations + 1];
  * status_out int[count] (1 ok, 0 non-positive curvature).  The host arrays are
  * read during the call (device tables are staged after draining the stream). */
+/* Batch autotuner: time the batch's interpolation steps (every problem's
+ * forward and adjoint launches) over the member group of frames sharing a
+ * pack (how many of them follow each work entry: 1, 2, 4, 8, 16 or all) and,
+ * when problems of 9..16 coils are present, over 1 or 2 waves per entry; keep
+ * the fastest.  Results are bitwise the same.  Call before capturing a graph
+ * (fails otherwise); synchronizes the stream.  The batch's interpolation
+ * buffers hold unspecified values afterwards (every solve rewrites them). */
+int tsn_batch_autotune(tsn_batch* batch, void* stream, uint32_t* best_member_group, uint32_t* best_coil_waves,
+                       float* best_us);
 int tsn_batch_cg(tsn_batch* batch, const void* const* measurements, uint32_t iterations,
                  const float* operator_scales, float lambda, void* const* x_out,
                  void* const* residuals_out, void* status_out, void* stream);

@@ -83,22 +83,37 @@ def test_repeat_launches_are_bitwise_identical():
 
 
 @pytest.mark.parametrize("view", ["G", "G_T"])
-@pytest.mark.parametrize("coils", [8, 4, 16])
-def test_fp32_csr_matches_oracle(view, coils):
+@pytest.mark.parametrize("coils", [8, 4, 16, 3])
+@pytest.mark.parametrize("spokes", [24, 96])
+def test_fp32_csr_matches_oracle(view, coils, spokes):
+    """FP32 CSR vs the float64 product.  With 96 spokes on a 32^2 image the
+    G_T rows near the k-space centre hold up to 687 entries: rows above 64
+    entries run as whole-wave chunks, rows above 256 as several chunks summed
+    through partial slots."""
+
     from oracle.solver import apply_csr, apply_csr_adjoint
     from trajsparsenufft_amd import device
-    _, packs = small_case()
+    _, packs = small_case(32, spokes)
     cols, re, im = packs.kb.arrays()
     d = device.Direction.csr(packs.kb, view)
     rng = np.random.default_rng(coils)
     x = (rng.standard_normal((d.cols, coils)) + 1j * rng.standard_normal((d.cols, coils))).astype(np.complex64)
-    out = torch.zeros((coils, d.rows), dtype=torch.complex64, device="cuda")
-    device.interp_f32(d, dev(x), out, coils=coils, conjugate=view == "G_T", scale=2.0)
     if view == "G":
         ref = 2.0 * apply_csr(cols, re, im, x.astype(np.complex128)).T
     else:
         ref = 2.0 * apply_csr_adjoint(cols, re, im, x.astype(np.complex128), d.rows).T
+    out = torch.zeros((coils, d.rows), dtype=torch.complex64, device="cuda")
+    device.interp_f32(d, dev(x), out, coils=coils, conjugate=view == "G_T", scale=2.0)
     assert rel_l2(out.cpu().numpy(), ref) <= 1e-6
+    # row-major layout, accumulate, and bitwise repeatability
+    rm = torch.zeros((d.rows, coils), dtype=torch.complex64, device="cuda")
+    device.interp_f32(d, dev(x), rm, coils=coils, conjugate=view == "G_T", scale=2.0, layout="row_major")
+    assert np.array_equal(np.ascontiguousarray(rm.cpu().numpy().T).view(np.uint32), out.cpu().numpy().view(np.uint32))
+    device.interp_f32(d, dev(x), out, coils=coils, conjugate=view == "G_T", scale=2.0, accumulate=True)
+    assert rel_l2(out.cpu().numpy(), 2.0 * ref) <= 1e-6
+    again = torch.zeros_like(rm)
+    device.interp_f32(d, dev(x), again, coils=coils, conjugate=view == "G_T", scale=2.0, layout="row_major")
+    assert np.array_equal(again.cpu().numpy().view(np.uint32), rm.cpu().numpy().view(np.uint32))
 
 
 @pytest.mark.parametrize("coils,rows", [(12, 1000), (5, 130), (16, 64), (3, 1)])
@@ -227,6 +242,45 @@ def test_interleaved_panel_path_is_bitwise_the_split_path(view, coils):
     device.interp_il(d, il, out_sp, coils=coils, conjugate=conj, scale=0.25, layout="split_il")
     ref_sp = device.split_panel_il(out.contiguous(), coils)
     assert np.array_equal(out_sp.cpu().numpy(), ref_sp.cpu().numpy())
+
+
+@pytest.mark.parametrize("coils", [12, 16])
+@pytest.mark.parametrize("layout", ["row_major", "split_il", "coil_major"])
+def test_coil_waves_are_bitwise_invariant(coils, layout):
+    """9..16 coils on interleaved panels: one wave for both coil blocks and
+    one wave per block give the same bits, long groups (chunks reduced through
+    slots by the last of both waves) included, and match the float64 product."""
+
+    from oracle.solver import apply_pack
+    from trajsparsenufft_amd import device
+    from trajsparsenufft_amd.device import Direction
+    _, packs = small_case(32, 96)
+    arrays, header = packs.adjoint.arrays(), packs.adjoint.header
+    assert np.diff(arrays["group_offsets"]).max() > 16
+    d = Direction.upload(packs.adjoint)
+    rng = np.random.default_rng(coils)
+    v = (rng.standard_normal((coils, header["n_cols"])) + 1j * rng.standard_normal((coils, header["n_cols"])))
+    il = device.split_panel_il(torch.from_numpy(v.astype(np.complex64)).cuda(), coils)
+    rows, blocks = header["output_size"], (coils + 7) // 8
+    outs = []
+    for waves in (1, 2, 1):
+        d.coil_waves = waves
+        assert d.coil_waves == waves
+        if layout == "split_il":
+            out = torch.zeros((blocks, rows, 16), dtype=torch.int32, device="cuda")
+        else:
+            shape = (rows, coils) if layout == "row_major" else (coils, rows)
+            out = torch.zeros(shape, dtype=torch.complex64, device="cuda")
+        device.interp_il(d, il, out, coils=coils, conjugate=True, scale=0.5, layout=layout)
+        outs.append(out.cpu().numpy())
+    assert all(np.array_equal(outs[0].view(np.uint32), o.view(np.uint32)) for o in outs[1:])
+    if layout != "split_il":
+        w = il.cpu().numpy().view(np.uint32)
+        hi = (w & 0xFFFF).astype(np.uint16).view(np.float16)
+        res = (w >> 16).astype(np.uint16).view(np.float16)
+        ref = apply_pack(arrays, rows, hi, res, coils, True, 0.5)
+        got = outs[0].T if layout == "row_major" else outs[0]
+        assert rel_l2(got, ref) <= TOL
 
 
 @pytest.mark.parametrize("view", ["G", "G_T"])

@@ -3,7 +3,15 @@
 ~ r^-1.5), 10 CG steps.  Every variant (FP32 plain, FP16 hi + residual on
 smfmac, BF16 hi + residual on smfmac, FP16 hi only, dense-MFMA control)
 reports its magnitude NRMSE against an FP64 direct-NUDFT CG and its
-forward/adjoint adjointness defect."""
+forward/adjoint adjointness defect.
+
+The discriminating gate is at the operator: each variant's plan forward and
+adjoint at this full shape against the float64 arithmetic model of that
+variant (``oracle.solver.KbOperator``: same KB coefficients, the variant's
+matrix and panel roundings), 2e-5 as the reference's native-vs-model gate
+(``validate_fp16x2_native.py:251-252``) and 1e-4 for the two variants whose
+BF16 or FP16-only rounding of an FP32 FFT value can fall on either side of a
+rounding boundary.  A 10x precision regression of any variant fails it."""
 
 import numpy as np
 import pytest
@@ -43,6 +51,29 @@ def spiral_case():
     x64 = ref.cg(torch.from_numpy(y).to(torch.complex128).cuda(), 10, 1e-4, 512).cpu().numpy()
     packs = pack.prepare_trajectory(traj, (256, 256), (512, 512))
     return packs, smaps, y, x64, x
+
+
+MODEL_MODES = {"fp32": ("fp32", "fp32"), "split16": ("fp16", "split2"), "dense16": ("fp16", "split2"),
+               "hi16": ("fp16", "fp16"), "splitbf16": ("bf16", "bf16split")}
+MODEL_TOL = {"fp32": 2e-5, "split16": 2e-5, "dense16": 2e-5, "hi16": 1e-4, "splitbf16": 1e-4}
+
+
+@pytest.mark.parametrize("precision", ["fp32", "split16", "splitbf16", "hi16", "dense16"])
+def test_spiral_operators_match_arithmetic_model(spiral_case, precision):
+    from oracle.nudft import complex_rel_l2
+    from oracle.solver import KbOperator
+    from trajsparsenufft_amd.recon import build_plan
+    packs, smaps, y, _, x = spiral_case
+    mp = packs.with_values("bf16") if precision == "splitbf16" else packs
+    cols, re, im = mp.kb.arrays()
+    model = KbOperator(cols, re, im, smaps, mp.apod, (256, 256), (512, 512), *MODEL_MODES[precision])
+    plan = build_plan(packs, dev(smaps), precision)
+    fwd = complex_rel_l2(plan.forward(dev(x)).cpu().numpy(), model.forward(x))
+    ys = (y / 512.0).astype(np.complex64)
+    adj = complex_rel_l2(plan.adjoint(dev(ys)).cpu().numpy(), model.adjoint(ys))
+    print(f"\n[sweep] spiral 16x8192 {precision}: operator vs model forward {fwd:.2e}, adjoint {adj:.2e}")
+    assert fwd <= MODEL_TOL[precision]
+    assert adj <= MODEL_TOL[precision]
 
 
 @pytest.mark.parametrize("precision", ["fp32", "split16", "splitbf16", "hi16", "dense16"])

@@ -44,3 +44,20 @@ def test_cpu_cg_matches_model_and_exact_nudft_cg():
     assert nudft.complex_rel_l2(got, emu) < 1e-5
     exact, _, _ = nudft.nudft_cg_sense(meas, smaps, omega, gsz, 8)
     assert nudft.magnitude_nrmse(got, exact) < 2e-4
+
+
+def test_omp_backend_equals_scipy_backend():
+    """The C CSR product (any thread count) adds each row in entry order, as
+    scipy's CSR product does: the operators agree to float32 rounding."""
+
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run(["make", "-s", "-C", os.path.join(root, "oracle")], check=True)
+    omega, cols, re, im, smaps, apod, x, isz, gsz = _case()
+    ref = CpuSense(cols, re, im, smaps, apod, isz, gsz)
+    y = ref.forward(x)
+    for threads in (1, 3, 8):
+        omp = CpuSense(cols, re, im, smaps, apod, isz, gsz, workers=threads, backend="omp")
+        assert nudft.complex_rel_l2(omp.forward(x), y) < 1e-6
+        assert nudft.complex_rel_l2(omp.adjoint(y), ref.adjoint(y)) < 1e-6

@@ -8,6 +8,7 @@
 //
 // Semantics per problem are exactly tsn_cg's (same kernels, same reduction
 // order), so a batched solve reproduces the single-plan solve bit for bit.
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -91,6 +92,8 @@ struct tsn_batch {
   std::vector<uint32_t> coils, m;
   std::vector<const tsn_direction*> fwd, adj;
   InterpBatch ib_fwd[2], ib_adj[2];  // parts of one and of two coil blocks
+  std::vector<InterpBatchPart> parts_fwd[2], parts_adj[2];  // (kept for the autotuner's rebuilds)
+  uint32_t member_group = 0, coil_waves = kDefaultCoilWaves;
   std::vector<void*> allocations;
   size_t bytes = 0;
   float2 *tw0 = nullptr, *tw1 = nullptr, *smaps_t = nullptr, *t_fwd = nullptr, *t_inv = nullptr, *grid_buf = nullptr;
@@ -303,9 +306,93 @@ int check_batch_call(tsn_batch* bt, const void* const* meas, uint32_t iterations
   return TSN_OK;
 }
 
+// (Re)build the batched interpolation tables with the batch's member group
+// and coil waves.
+int build_interp_tables(tsn_batch* bt) {
+  for (int k = 0; k < 2; ++k) {
+    interp_batch_free(&bt->ib_fwd[k]);
+    interp_batch_free(&bt->ib_adj[k]);
+  }
+  for (int k = 0; k < 2; ++k) {
+    if (bt->parts_fwd[k].empty()) continue;
+    TSN_CHECK(interp_batch_build(bt->parts_fwd[k], &bt->ib_fwd[k], bt->member_group, bt->coil_waves));
+    TSN_CHECK(interp_batch_build(bt->parts_adj[k], &bt->ib_adj[k], bt->member_group, bt->coil_waves));
+  }
+  return TSN_OK;
+}
+
+// One forward and one adjoint interpolation step of every problem.
+int enqueue_interp_steps(tsn_batch* bt, hipStream_t s) {
+  const int kind = interp_kind_of(bt->cfg.precision);
+  const float norm = static_cast<float>(1.0 / std::sqrt(static_cast<double>(bt->cfg.grid0) * bt->cfg.grid1));
+  for (const InterpBatch& ib : bt->ib_fwd)
+    if (ib.n_problems) TSN_CHECK(launch_interp_batch(ib, kind, 3, false, norm, s));
+  for (const InterpBatch& ib : bt->ib_adj)
+    if (ib.n_problems) TSN_CHECK(launch_interp_batch(ib, kind, 1, true, 1.0f, s));
+  return TSN_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int tsn_batch_autotune(tsn_batch* bt, void* stream, uint32_t* best_member_group, uint32_t* best_coil_waves,
+                       float* best_us) {
+  if (!bt) return fail(TSN_ERR_ARGUMENT, "null batch");
+  if (bt->graph) return fail(TSN_ERR_ARGUMENT, "tune a batch before capturing its graph");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  TSN_HIP(hipSetDevice(bt->device));
+  TSN_HIP(hipStreamSynchronize(s));
+  std::vector<uint32_t> groups;
+  for (uint32_t g : {1u, 2u, 4u, 8u, 16u, bt->problems})
+    if (g <= bt->problems && std::find(groups.begin(), groups.end(), g) == groups.end()) groups.push_back(g);
+  std::vector<uint32_t> waves = {bt->coil_waves};
+  if (!bt->parts_fwd[1].empty()) waves = {1u, 2u};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = TSN_OK;
+  auto hip = [&](hipError_t e, const char* what) {
+    if (rc == TSN_OK && e != hipSuccess) rc = fail(TSN_ERR_RUNTIME, "batch autotune %s: %s", what, hipGetErrorString(e));
+  };
+  hip(hipEventCreate(&e0), "event");
+  hip(hipEventCreate(&e1), "event");
+  uint32_t best_g = 0, best_w = 0;
+  float best_ms = 0.0f;
+  for (uint32_t w : waves) {
+    for (uint32_t g : groups) {
+      if (rc != TSN_OK) break;
+      bt->member_group = g;
+      bt->coil_waves = w;
+      rc = build_interp_tables(bt);
+      for (int i = 0; rc == TSN_OK && i < 2; ++i) rc = enqueue_interp_steps(bt, s);
+      if (rc != TSN_OK) break;
+      hip(hipEventRecord(e0, s), "event");
+      constexpr int kReps = 4;
+      for (int i = 0; rc == TSN_OK && i < kReps; ++i) rc = enqueue_interp_steps(bt, s);
+      hip(hipEventRecord(e1, s), "event");
+      hip(hipEventSynchronize(e1), "event");
+      float ms = 0.0f;
+      hip(hipEventElapsedTime(&ms, e0, e1), "event");
+      ms /= kReps;
+      if (rc == TSN_OK && (best_g == 0 || ms < best_ms)) {
+        best_g = g;
+        best_w = w;
+        best_ms = ms;
+      }
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc != TSN_OK) return rc;
+  bt->member_group = best_g;
+  bt->coil_waves = best_w;
+  TSN_CHECK(build_interp_tables(bt));
+  TSN_HIP(hipStreamSynchronize(s));
+  if (best_member_group) *best_member_group = best_g;
+  if (best_coil_waves) *best_coil_waves = best_w;
+  if (best_us) *best_us = best_ms * 1000.0f;
+  return TSN_OK;
+}
 
 int tsn_batch_create(const tsn_plan_config* cfg, const tsn_batch_problem* problems, uint32_t count,
                      const void* apodization, tsn_batch** out) {
@@ -414,11 +501,11 @@ int tsn_batch_create(const tsn_plan_config* cfg, const tsn_batch_problem* proble
       fw[blocks - 1].push_back({bt->fwd[z], pan, nullptr, smp, bt->cmax, 0, blocks});
       ad[blocks - 1].push_back({bt->adj[z], smp, grid_out, nullptr, bt->cmax, 0, blocks});
     }
-    for (int k = 0; k < 2 && rc == TSN_OK; ++k) {
-      if (fw[k].empty()) continue;
-      rc = interp_batch_build(fw[k], &bt->ib_fwd[k]);
-      if (rc == TSN_OK) rc = interp_batch_build(ad[k], &bt->ib_adj[k]);
+    for (int k = 0; k < 2; ++k) {
+      bt->parts_fwd[k] = fw[k];
+      bt->parts_adj[k] = ad[k];
     }
+    rc = build_interp_tables(bt);
   }
   if (rc != TSN_OK) {
     batch_destroy(bt);

@@ -241,6 +241,16 @@ __device__ __forceinline__ void dft8(const float2* a, float2* X) {
   X[7] = csub(E[3], o3);
 }
 
+// Orders one wave's LDS accesses around this point for data that only this
+// wave writes and reads (its own slice): a wave's LDS instructions execute in
+// issue order, so keeping the compiler from moving accesses across the point
+// is enough; the other waves of the block run on without waiting.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Per-wave LDS slice (float2 elements): pass-A layout R rows of 65, pass-B/C
 // layout (R*8 tasks) x 9; both padded against bank conflicts.
 template <int R>
@@ -275,8 +285,9 @@ __device__ __forceinline__ WaveTwiddles<R> load_twiddles(const float2* __restric
 // lane + 64 j (elements >= K/2 are zero when HALF_IN; the caller loads zeros
 // there, and for even R the radix-R pass skips them).  Output: out[s][k2b] = X[k1 + R*(k2a + 8*k2b)]
 // for task = lane + 64 s < 8R, k1 = task >> 3, k2a = task & 7 (only k2b < 4,
-// i.e. k < K/2, when HALF_OUT).  tw = W_K^k table.  Uses __syncthreads: all
-// waves of the block must call it together.
+// i.e. k < K/2, when HALF_OUT).  tw = W_K^k table.  buf is the wave's own
+// slice: the passes synchronize the wave only, so the caller orders the
+// slice against other waves' use of the same LDS (barriers before and after).
 template <int R, bool INV, bool HALF_IN, bool HALF_OUT>
 __device__ __forceinline__ void wave_fft(float2 (&x)[R], float2* buf, const float2* __restrict__ tw,
                                          const WaveTwiddles<R>& twp, int lane,
@@ -308,7 +319,7 @@ __device__ __forceinline__ void wave_fft(float2 (&x)[R], float2* buf, const floa
   for (int k1 = 1; k1 < R; ++k1) y[k1] = twiddle<INV>(y[k1], R <= 8 ? twp.a[k1] : tw[lane * k1]);
 #pragma unroll
   for (int k1 = 0; k1 < R; ++k1) buf[k1 * 65 + lane] = y[k1];
-  __syncthreads();
+  wave_lds_sync();
   // pass B: task (k1, b0): radix-8 over b1, twiddle W_64^{b0*k2a}
   float2 u[TPL][8];
 #pragma unroll
@@ -320,7 +331,7 @@ __device__ __forceinline__ void wave_fft(float2 (&x)[R], float2* buf, const floa
       for (int b1 = 0; b1 < 8; ++b1) u[s][b1] = buf[k1 * 65 + b0 + 8 * b1];
     }
   }
-  __syncthreads();
+  wave_lds_sync();
 #pragma unroll
   for (int s = 0; s < TPL; ++s) {
     const int task = lane + 64 * s;
@@ -334,7 +345,7 @@ __device__ __forceinline__ void wave_fft(float2 (&x)[R], float2* buf, const floa
       for (int k2a = 0; k2a < 8; ++k2a) buf[(k1 * 8 + k2a) * 9 + b0] = v[k2a];
     }
   }
-  __syncthreads();
+  wave_lds_sync();
   // pass C: task (k1, k2a): radix-8 over b0
 #pragma unroll
   for (int s = 0; s < TPL; ++s) {

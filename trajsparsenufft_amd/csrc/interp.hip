@@ -477,19 +477,23 @@ constexpr uint32_t kSlotBatch = 8;  // slots loaded before the first add (one ro
 // Slot byte offsets are 32-bit buffer offsets: at most 2^31 bytes of slots.
 constexpr uint32_t kMaxSlots = (1u << 31) / (kSlotFloats * 4);
 
-template <int OM, bool CONJ, bool ACC, int NB>
+// WB > 1 (one wave per coil block of the entry): wave wb publishes its
+// accumulators as block wb, the group's last arrival of all chunks and waves
+// adds the slots of every block; a holds the unshifted (block 0) arguments.
+template <int OM, bool CONJ, bool ACC, int NB, int WB = 1>
 __device__ __forceinline__ void chunk_finish(const InterpArgs& a, const ChunkItem& c, float4_t (&acc_re)[NB],
-                                             float4_t (&acc_im)[NB], int lane) {
+                                             float4_t (&acc_im)[NB], int lane, int wb = 0) {
   // slot layout: [coil block][re, im][lane][4] floats; slot offsets stay below
   // 2^32 bytes (kSlotFloats * 4 bytes per chunk)
+  static_assert(NB * WB <= 2, "a slot holds two coil blocks");
   const rsrc_t slots = make_rsrc(a.partials, uint64_t{1} << 32);
   const int mine = static_cast<int>((c.slot0 + c.chunk) * kSlotFloats * 4);
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, acc_re[b]), slots,
-                                           ((b * 2 + 0) * 64 + lane) * 16, mine, kCpolDevice);
+                                           (((b + wb) * 2 + 0) * 64 + lane) * 16, mine, kCpolDevice);
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, acc_im[b]), slots,
-                                           ((b * 2 + 1) * 64 + lane) * 16, mine, kCpolDevice);
+                                           (((b + wb) * 2 + 1) * 64 + lane) * 16, mine, kCpolDevice);
   }
   __builtin_amdgcn_s_waitcnt(0);  // every slot store acknowledged
   asm volatile("" ::: "memory");
@@ -497,14 +501,14 @@ __device__ __forceinline__ void chunk_finish(const InterpArgs& a, const ChunkIte
   if (lane == 0)
     arrived = __hip_atomic_fetch_add(a.counters + c.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   arrived = __builtin_amdgcn_readfirstlane(arrived);
-  if (arrived + 1 != c.n_chunks) return;
+  if (arrived + 1 != c.n_chunks * WB) return;
   asm volatile("" ::: "memory");
   if (lane == 0) __hip_atomic_store(a.counters + c.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int first = static_cast<int>(c.slot0 * kSlotFloats * 4);
   const int4 rows = group_rows_of(a, static_cast<uint32_t>(c.group), lane);
   const uint32_t n = c.n_chunks;
 #pragma unroll
-  for (int b = 0; b < NB; ++b) {
+  for (int b = 0; b < NB * WB; ++b) {
     float4_t re = float4_t{0.f, 0.f, 0.f, 0.f}, im = re;
     // kSlotBatch slots are loaded (all before the first add), then added in
     // chunk order: ceil(n / kSlotBatch) memory round trips
@@ -540,10 +544,17 @@ __device__ __forceinline__ void chunk_finish(const InterpArgs& a, const ChunkIte
 // entry whenever a wave slot frees, so uneven entries balance themselves
 // across the CUs.  Units and chunks share one tile loop (one inlined copy
 // keeps the register count of the regular units).
-template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC>
-__global__ __launch_bounds__(64) void interp_tiles_kernel(InterpArgs a) {
-  __shared__ uint32_t stage[kStageWords];
-  const int lane = threadIdx.x;
+// WB = 2 (entries of two coil blocks, NB = 1): a block of two waves per
+// entry, wave w computing coil block w.  Both stream the entry's tile operands
+// (the second read comes from cache: the two waves share a CU), each with the
+// one-block register budget and pipeline, and gather only its own panel.
+template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC, int WB = 1>
+__global__ __launch_bounds__(64 * WB) void interp_tiles_kernel(InterpArgs a) {
+  static_assert(WB == 1 || NB == 1, "one wave per coil block covers one block");
+  __shared__ uint32_t stages[WB][kStageWords];
+  const int lane = threadIdx.x & 63;
+  const int wb = WB > 1 ? static_cast<int>(threadIdx.x >> 6) : 0;
+  uint32_t* stage = stages[wb];
   float4_t acc_re[NB], acc_im[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -567,17 +578,29 @@ __global__ __launch_bounds__(64) void interp_tiles_kernel(InterpArgs a) {
   if constexpr (IL) {
     if (a.problems) v = problem_args(a, problem);
   }
+  // this wave's coil block: input panel, output coils and split-panel block
+  InterpArgs vw = v;
+  if constexpr (WB > 1) {
+    vw.panel_il = v.panel_il ? v.panel_il + wb * v.panel_stride : nullptr;
+    vw.panel_hi = v.panel_hi ? v.panel_hi + wb * v.panel_stride : nullptr;
+    vw.panel_res = v.panel_res ? v.panel_res + wb * v.panel_stride : nullptr;
+    vw.coil_base = v.coil_base + wb * kCoilsPerBlock;
+    const size_t out_block = static_cast<size_t>(wb) * v.n_out * kN;
+    vw.out_il = v.out_il ? v.out_il + out_block : nullptr;
+    vw.out_hi = v.out_hi ? v.out_hi + out_block : nullptr;
+    vw.out_res = v.out_res ? v.out_res + out_block : nullptr;
+  }
   if (u.tile_begin == u.tile_end) {  // empty group: zero rows
-    const int4 rows = group_rows_of(v, static_cast<uint32_t>(u.group), lane);
+    const int4 rows = group_rows_of(vw, static_cast<uint32_t>(u.group), lane);
 #pragma unroll
-    for (int b = 0; b < NB; ++b) interp_epilogue<OM, CONJ, ACC>(v, rows, b, acc_re[b], acc_im[b], lane);
+    for (int b = 0; b < NB; ++b) interp_epilogue<OM, CONJ, ACC>(vw, rows, b, acc_re[b], acc_im[b], lane);
     return;
   }
-  const TileRsrc<NB> r = make_tile_rsrc<NB, IL>(v);
+  const TileRsrc<NB> r = make_tile_rsrc<NB, IL>(vw);
   stage_cols(r, u.tile_begin, u.tile_end - u.tile_begin, stage, lane);
-  stream_tiles<KIND, NB, IL, OM, CONJ, ACC>(v, r, u.tile_begin, u.tile_end, u.group, u.group_end, is_chunk ? 0u : 1u,
-                                            acc_re, acc_im, lane, stage);
-  if (is_chunk) chunk_finish<OM, CONJ, ACC, NB>(v, c, acc_re, acc_im, lane);
+  stream_tiles<KIND, NB, IL, OM, CONJ, ACC>(vw, r, u.tile_begin, u.tile_end, u.group, u.group_end,
+                                            is_chunk ? 0u : 1u, acc_re, acc_im, lane, stage);
+  if (is_chunk) chunk_finish<OM, CONJ, ACC, NB, WB>(v, c, acc_re, acc_im, lane, wb);
 }
 
 
@@ -586,106 +609,177 @@ struct CsrArgs {
   const int32_t* row_ids;
   const int32_t* cols;
   const float2* vals;
-  const int32_t* short_rows;
-  uint32_t n_short;
-  const int32_t* long_rows;
-  uint32_t n_long;  // multiple of 8 (padding entries are -1)
+  const CsrItem* items;
+  uint32_t n_items;
+  float2* partials;    // [slots][kMaxCoils] partial sums of long-row chunks
+  uint32_t* counters;  // one arrival counter per row of several chunks, zero between launches
   const float2* in;
+  uint32_t n_cols;
   uint32_t coils;
   uint32_t n_out;
   float scale;
   float2* out;
 };
 
-// Plain FP32 path: LPR lanes per CSR row (one coil each), FP32 complex
-// coefficients, input rows float2 [row][coils].  Rows longer than
-// kLongRowEntries get a whole block: 256/LPR lane groups stride over the
-// entries and are summed in group order through LDS.
+// Plain FP32 path.  One wave per block takes one work item: a run of at most
+// kCsrRowsPerItem consecutive short rows (each at most kCsrShortEntries
+// entries), or one chunk of a longer row (at most kCsrChunkEntries entries).
+// The item's column ids and FP32 complex coefficients are contiguous in the
+// CSR arrays; the wave copies them into LDS with coalesced loads, all issued
+// before the first wait.  LPR lanes (one coil each) then work on a row: the
+// lane groups of a run take its rows, the lane groups of a chunk stride its
+// entries and are summed in a fixed butterfly order.  Each group reads
+// kCsrBatch (column, value) pairs from LDS (one broadcast address per group),
+// issues their kCsrBatch gathers of input rows float2 [row][coils] and then
+// accumulates in entry order, so a row's sum never depends on the schedule.
+// A row of several chunks publishes one partial per chunk to a slot, and the
+// wave that counts the row's last arrival adds the slots in chunk order and
+// writes the row (the slot protocol of the packed kernel's long groups).
+constexpr int kCsrBatch = 16;
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+typedef float float2_t __attribute__((ext_vector_type(2)));
+
 template <bool CONJ>
-__device__ __forceinline__ void csr_accumulate(const CsrArgs& a, uint32_t k, uint32_t c, bool active,
-                                               float& re, float& im) {
-  const float2 v = a.vals[k];
-  const float2 x = active ? a.in[static_cast<size_t>(a.cols[k]) * a.coils + c] : make_float2(0.f, 0.f);
+__device__ __forceinline__ void cmac(float2_t v, float2_t x, bool valid, float& re, float& im) {
+  float pr, pi;
   if (CONJ) {
-    re += v.x * x.x + v.y * x.y;
-    im += v.x * x.y - v.y * x.x;
+    pr = v.x * x.x + v.y * x.y;
+    pi = v.x * x.y - v.y * x.x;
   } else {
-    re += v.x * x.x - v.y * x.y;
-    im += v.x * x.y + v.y * x.x;
+    pr = v.x * x.x - v.y * x.y;
+    pi = v.x * x.y + v.y * x.x;
+  }
+  re = valid ? re + pr : re;
+  im = valid ? im + pi : im;
+}
+
+// Entries first, first + step, ... < end of the LDS stage, kCsrBatch at a time.
+template <bool CONJ>
+__device__ __forceinline__ void csr_row_sum(const int32_t* s_col, const float2_t* s_val, uint32_t first, uint32_t end,
+                                            uint32_t step, rsrc_t in, uint32_t lane_off, uint32_t row_bytes,
+                                            float& re, float& im) {
+  for (uint32_t e0 = first; e0 < end; e0 += kCsrBatch * step) {
+    float2_t v[kCsrBatch], x[kCsrBatch];
+    bool ok[kCsrBatch];
+#pragma unroll
+    for (int i = 0; i < kCsrBatch; ++i) {
+      const uint32_t e = e0 + i * step;
+      ok[i] = e < end;
+      const uint32_t ec = ok[i] ? e : first;
+      v[i] = s_val[ec];
+      const uint32_t col = static_cast<uint32_t>(s_col[ec]);
+      x[i] = __builtin_bit_cast(float2_t, __builtin_amdgcn_raw_buffer_load_b64(in, col * row_bytes + lane_off, 0, 0));
+    }
+#pragma unroll
+    for (int i = 0; i < kCsrBatch; ++i) cmac<CONJ>(v[i], x[i], ok[i], re, im);
   }
 }
 
 template <int OM, bool ACC>
 __device__ __forceinline__ void csr_store(const CsrArgs& a, int32_t row, uint32_t c, float re, float im) {
-  float2 v = make_float2(re * a.scale, im * a.scale);
+  float2_t v = float2_t{re * a.scale, im * a.scale};
   size_t o;
   if constexpr (OM == kOutCoilMajor) o = static_cast<size_t>(c) * a.n_out + row;
   else o = static_cast<size_t>(row) * a.coils + c;
-  if constexpr (ACC) {
-    const float2 prior = a.out[o];
-    v.x += prior.x;
-    v.y += prior.y;
-  }
-  a.out[o] = v;
+  float2_t* out = reinterpret_cast<float2_t*>(a.out);
+  if constexpr (ACC) v += glb(out)[o];
+  glb(out)[o] = v;
 }
 
 template <int LPR, int OM, bool CONJ, bool ACC>
-__global__ __launch_bounds__(256) void interp_csr_kernel(CsrArgs a) {
-  constexpr int kGroups = 256 / LPR;
-  __shared__ float2 red[256];
-  const uint32_t seg = threadIdx.x / LPR, c = threadIdx.x % LPR;
+__global__ __launch_bounds__(64) void interp_csr_kernel(CsrArgs a) {
+  constexpr int G = 64 / LPR;  // lane groups per wave
+  __shared__ int32_t s_col[kCsrStageEntries];
+  __shared__ float2_t s_val[kCsrStageEntries];
+  const uint32_t lane = threadIdx.x;
+  const CsrItem it = a.items[xcd_remap(blockIdx.x, gridDim.x)];
+  const uint32_t n = it.e_end - it.e_begin;
+  const uint32_t g = lane / LPR, c = lane % LPR;
   const bool active = c < a.coils;
-  if (blockIdx.x < a.n_long) {
-    const int32_t task = a.long_rows[blockIdx.x];
-    if (task < 0) return;  // padding block: uniform exit before the barrier
-    float re0 = 0.f, im0 = 0.f;
-    for (uint32_t k = a.row_ptr[task] + seg; k < a.row_ptr[task + 1]; k += kGroups)
-      csr_accumulate<CONJ>(a, k, c, active, re0, im0);
-    red[threadIdx.x] = make_float2(re0, im0);
-    __syncthreads();
-    if (seg != 0 || !active) return;
-    float re = 0.f, im = 0.f;
-    for (int g = 0; g < kGroups; ++g) {
-      re += red[g * LPR + c].x;
-      im += red[g * LPR + c].y;
-    }
-    csr_store<OM, ACC>(a, a.row_ids[task], c, re, im);
-    return;
+  const bool chunk = it.n_chunks != 0;
+  // the group's row of a run (its first; runs hold at most G rows when LPR <= 8)
+  const uint32_t row = chunk ? it.row_begin : min(it.row_begin + g, it.row_end - 1);
+  const uint32_t rb = glb(a.row_ptr)[row], re_ = glb(a.row_ptr)[row + 1];
+  const int32_t out_row = glb(a.row_ids)[row];
+  // stage the item's entries (clamped loads, every one issued before the first wait)
+  int32_t cbuf[kCsrStageEntries / 64];
+  float2_t vbuf[kCsrStageEntries / 64];
+#pragma unroll
+  for (int i = 0; i < kCsrStageEntries / 64; ++i) {
+    const uint32_t e = it.e_begin + min(i * 64 + lane, n - 1);
+    cbuf[i] = glb(a.cols)[e];
+    vbuf[i] = glb(reinterpret_cast<const float2_t*>(a.vals))[e];
   }
-  const uint32_t rb = blockIdx.x - a.n_long;
-  const uint32_t idx = xcd_remap(rb, gridDim.x - a.n_long) * kGroups + seg;
-  if (idx >= a.n_short) return;
-  const int32_t task = a.short_rows[idx];
-  const uint32_t begin = a.row_ptr[task], end = a.row_ptr[task + 1];
-  const uint32_t cc = active ? c : 0;  // inactive lanes read coil 0 and discard it
-  float re0 = 0.f, im0 = 0.f, re1 = 0.f, im1 = 0.f;
-  uint32_t k = begin;
-  for (; k + 4 <= end; k += 4) {
-    int32_t col[4];
-    float2 v[4], x[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      col[i] = a.cols[k + i];
-      v[i] = a.vals[k + i];
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) x[i] = a.in[static_cast<size_t>(col[i]) * a.coils + cc];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      float& re = (i & 1) ? re1 : re0;
-      float& im = (i & 1) ? im1 : im0;
-      if (CONJ) {
-        re += v[i].x * x[i].x + v[i].y * x[i].y;
-        im += v[i].x * x[i].y - v[i].y * x[i].x;
-      } else {
-        re += v[i].x * x[i].x - v[i].y * x[i].y;
-        im += v[i].x * x[i].y + v[i].y * x[i].x;
+  for (int i = 0; i < kCsrStageEntries / 64; ++i) {
+    s_col[i * 64 + lane] = cbuf[i];
+    s_val[i * 64 + lane] = vbuf[i];
+  }
+  __syncthreads();
+  const uint32_t row_bytes = a.coils * 8;
+  const rsrc_t in = make_rsrc(a.in, static_cast<uint64_t>(a.n_cols) * row_bytes);
+  const uint32_t lane_off = (active ? c : 0) * 8;
+  float re = 0.f, im = 0.f;
+  if (!chunk) {
+    uint32_t r = it.row_begin + g, b = rb, e = re_;
+    int32_t orow = out_row;
+    while (r < it.row_end) {
+      re = 0.f;
+      im = 0.f;
+      csr_row_sum<CONJ>(s_col, s_val, b - it.e_begin, e - it.e_begin, 1, in, lane_off, row_bytes, re, im);
+      if (active) csr_store<OM, ACC>(a, orow, c, re, im);
+      r += G;
+      if (r < it.row_end) {
+        b = glb(a.row_ptr)[r];
+        e = glb(a.row_ptr)[r + 1];
+        orow = glb(a.row_ids)[r];
       }
     }
+    return;
   }
-  for (; k < end; ++k) csr_accumulate<CONJ>(a, k, cc, true, re0, im0);
-  if (!active) return;
-  csr_store<OM, ACC>(a, a.row_ids[task], c, re0 + re1, im0 + im1);
+  // one chunk of a long row: the groups stride the entries, then a butterfly over the groups
+  csr_row_sum<CONJ>(s_col, s_val, g, n, G, in, lane_off, row_bytes, re, im);
+#pragma unroll
+  for (int off = LPR; off < 64; off <<= 1) {
+    re += __shfl_xor(re, off);
+    im += __shfl_xor(im, off);
+  }
+  if (it.n_chunks == 1) {
+    if (g == 0 && active) csr_store<OM, ACC>(a, out_row, c, re, im);
+    return;
+  }
+  const rsrc_t slots = make_rsrc(a.partials, uint64_t{1} << 32);
+  if (g == 0 && active)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, float2_t{re, im}), slots, c * 8,
+                                          static_cast<int>((it.slot0 + it.chunk) * kMaxCoils * 8), kCpolDevice);
+  __builtin_amdgcn_s_waitcnt(0);  // the slot stores acknowledged
+  asm volatile("" ::: "memory");
+  uint32_t arrived = 0;
+  if (lane == 0)
+    arrived = __hip_atomic_fetch_add(a.counters + it.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  arrived = __builtin_amdgcn_readfirstlane(arrived);
+  if (arrived + 1 != it.n_chunks) return;
+  asm volatile("" ::: "memory");
+  if (lane == 0) __hip_atomic_store(a.counters + it.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (g != 0 || !active) return;
+  float sr = 0.f, si = 0.f;
+  for (uint32_t q0 = 0; q0 < it.n_chunks; q0 += kSlotBatch) {
+    float2_t p[kSlotBatch];
+#pragma unroll
+    for (uint32_t j = 0; j < kSlotBatch; ++j) {
+      const uint32_t q = min(q0 + j, it.n_chunks - 1);
+      p[j] = __builtin_bit_cast(float2_t, __builtin_amdgcn_raw_buffer_load_b64(
+                                            slots, c * 8, static_cast<int>((it.slot0 + q) * kMaxCoils * 8),
+                                            kCpolDevice));
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kSlotBatch; ++j) {
+      if (q0 + j >= it.n_chunks) break;
+      sr += p[j].x;
+      si += p[j].y;
+    }
+  }
+  csr_store<OM, ACC>(a, out_row, c, sr, si);
 }
 
 // hi = RN16(v), res = RN16(v - hi), v = in * scale; in float2 [coil][rows];
@@ -773,34 +867,35 @@ __global__ __launch_bounds__(256) void dc_residual_kernel(const float* __restric
 
 // ------------------------------------------------------------- dispatch
 
-template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC>
+template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC, int WB>
 static void launch_both(const InterpArgs& a, hipStream_t s) {
   const uint32_t blocks = a.n_chunks + a.n_items;
-  if (blocks) hipLaunchKernelGGL((interp_tiles_kernel<KIND, NB, IL, OM, CONJ, ACC>), dim3(blocks), dim3(64), 0, s, a);
+  if (blocks)
+    hipLaunchKernelGGL((interp_tiles_kernel<KIND, NB, IL, OM, CONJ, ACC, WB>), dim3(blocks), dim3(64 * WB), 0, s, a);
 }
 
-template <int KIND, int NB, bool IL, int OM>
+template <int KIND, int NB, bool IL, int OM, int WB = 1>
 static void launch_tiles_om(const InterpArgs& a, bool conj, bool acc, hipStream_t s) {
   constexpr bool kAccOk = OM != kOutSplit16 && OM != kOutSplit16IL && OM != kOutSplitBF16IL;
   if (conj) {
-    if (kAccOk && acc) launch_both<KIND, NB, IL, OM, true, kAccOk>(a, s);
-    else launch_both<KIND, NB, IL, OM, true, false>(a, s);
+    if (kAccOk && acc) launch_both<KIND, NB, IL, OM, true, kAccOk, WB>(a, s);
+    else launch_both<KIND, NB, IL, OM, true, false, WB>(a, s);
   } else {
-    if (kAccOk && acc) launch_both<KIND, NB, IL, OM, false, kAccOk>(a, s);
-    else launch_both<KIND, NB, IL, OM, false, false>(a, s);
+    if (kAccOk && acc) launch_both<KIND, NB, IL, OM, false, kAccOk, WB>(a, s);
+    else launch_both<KIND, NB, IL, OM, false, false, WB>(a, s);
   }
 }
 
 // Interleaved panels (plan-internal) pair with FP32 or interleaved split
 // output; separate panels with FP32 or separate split output (checked by
-// launch_interp).
-template <int KIND, int NB, bool IL>
+// launch_interp).  WB = 2: two coil blocks, one wave each (NB = 1).
+template <int KIND, int NB, bool IL, int WB = 1>
 static void launch_tiles_nb(const InterpArgs& a, int om, bool conj, bool acc, hipStream_t s) {
   if constexpr (IL) {
-    if (om == kOutCoilMajor) launch_tiles_om<KIND, NB, IL, kOutCoilMajor>(a, conj, acc, s);
-    else if (om == kOutRowMajor) launch_tiles_om<KIND, NB, IL, kOutRowMajor>(a, conj, acc, s);
-    else if (KIND == kSparseSplitBF16) launch_tiles_om<KIND, NB, IL, kOutSplitBF16IL>(a, conj, false, s);
-    else launch_tiles_om<KIND, NB, IL, kOutSplit16IL>(a, conj, false, s);
+    if (om == kOutCoilMajor) launch_tiles_om<KIND, NB, IL, kOutCoilMajor, WB>(a, conj, acc, s);
+    else if (om == kOutRowMajor) launch_tiles_om<KIND, NB, IL, kOutRowMajor, WB>(a, conj, acc, s);
+    else if (KIND == kSparseSplitBF16) launch_tiles_om<KIND, NB, IL, kOutSplitBF16IL, WB>(a, conj, false, s);
+    else launch_tiles_om<KIND, NB, IL, kOutSplit16IL, WB>(a, conj, false, s);
   } else {
     if (om == kOutCoilMajor) launch_tiles_om<KIND, NB, IL, kOutCoilMajor>(a, conj, acc, s);
     else if (om == kOutRowMajor) launch_tiles_om<KIND, NB, IL, kOutRowMajor>(a, conj, acc, s);
@@ -808,16 +903,26 @@ static void launch_tiles_nb(const InterpArgs& a, int om, bool conj, bool acc, hi
   }
 }
 
+// Interleaved panels of two coil blocks: one wave for both blocks
+// (coil_waves 1) or one wave per block (coil_waves 2).
 template <int KIND>
-static void launch_tiles(const InterpArgs& a, bool il, int om, bool conj, bool acc, hipStream_t s) {
+static void launch_tiles_il2(const InterpArgs& a, int om, bool conj, bool acc, uint32_t coil_waves,
+                             hipStream_t s) {
+  if (coil_waves == 2) launch_tiles_nb<KIND, 1, true, 2>(a, om, conj, acc, s);
+  else launch_tiles_nb<KIND, 2, true>(a, om, conj, acc, s);
+}
+
+template <int KIND>
+static void launch_tiles(const InterpArgs& a, bool il, int om, bool conj, bool acc, uint32_t coil_waves,
+                         hipStream_t s) {
   const bool two = a.coils > kCoilsPerBlock;
   if constexpr (KIND == kSparseSplitBF16) {  // interleaved panels only (checked by launch_interp)
-    if (two) launch_tiles_nb<KIND, 2, true>(a, om, conj, acc, s);
+    if (two) launch_tiles_il2<KIND>(a, om, conj, acc, coil_waves, s);
     else launch_tiles_nb<KIND, 1, true>(a, om, conj, acc, s);
     return;
   }
   if (il) {
-    if (two) launch_tiles_nb<KIND, 2, true>(a, om, conj, acc, s);
+    if (two) launch_tiles_il2<KIND>(a, om, conj, acc, coil_waves, s);
     else launch_tiles_nb<KIND, 1, true>(a, om, conj, acc, s);
   } else {
     if (two) launch_tiles_nb<KIND, 2, false>(a, om, conj, acc, s);
@@ -880,11 +985,12 @@ static int launch_interp_impl(const tsn_direction* d, int kind, const void* hi, 
     a.dc_row_stride = dc->row_stride;
     a.dc_coil_stride = dc->coil_stride;
   }
-  if (kind == kSparseSplit) launch_tiles<kSparseSplit>(a, interleaved, out_mode, conj, accumulate, stream);
-  else if (kind == kDenseSplit) launch_tiles<kDenseSplit>(a, interleaved, out_mode, conj, accumulate, stream);
+  const uint32_t cw = d->coil_waves;
+  if (kind == kSparseSplit) launch_tiles<kSparseSplit>(a, interleaved, out_mode, conj, accumulate, cw, stream);
+  else if (kind == kDenseSplit) launch_tiles<kDenseSplit>(a, interleaved, out_mode, conj, accumulate, cw, stream);
   else if (kind == kSparseSplitBF16)
-    launch_tiles<kSparseSplitBF16>(a, interleaved, out_mode, conj, accumulate, stream);
-  else launch_tiles<kSparseHi>(a, interleaved, out_mode, conj, accumulate, stream);
+    launch_tiles<kSparseSplitBF16>(a, interleaved, out_mode, conj, accumulate, cw, stream);
+  else launch_tiles<kSparseHi>(a, interleaved, out_mode, conj, accumulate, cw, stream);
   TSN_HIP(hipGetLastError());
   return TSN_OK;
 }
@@ -953,13 +1059,9 @@ int launch_dc_residual_rows(float2* rows_io, uint32_t rows, uint32_t coils, cons
 }
 
 template <int LPR, int OM>
-static void launch_csr_om(const tsn_direction* d, const float2* in, uint32_t coils, bool conj,
-                          bool acc, float scale, float2* out, hipStream_t s) {
-  CsrArgs a{d->row_ptr, d->row_ids, d->cols, d->vals, d->short_rows, d->n_short, d->long_rows,
-            d->n_long_rows, in, coils, d->h.output_size, scale, out};
-  const uint32_t blocks = d->n_long_rows + div_up(static_cast<uint64_t>(d->n_short) * LPR, 256);
+static void launch_csr_om(const CsrArgs& a, bool conj, bool acc, hipStream_t s) {
 #define TSN_CSR(C, A) \
-  hipLaunchKernelGGL((interp_csr_kernel<LPR, OM, C, A>), dim3(blocks), dim3(256), 0, s, a)
+  hipLaunchKernelGGL((interp_csr_kernel<LPR, OM, C, A>), dim3(a.n_items), dim3(64), 0, s, a)
   if (conj) {
     if (acc) TSN_CSR(true, true); else TSN_CSR(true, false);
   } else {
@@ -969,23 +1071,43 @@ static void launch_csr_om(const tsn_direction* d, const float2* in, uint32_t coi
 }
 
 template <int LPR>
-static void launch_csr_lpr(const tsn_direction* d, const float2* in, uint32_t coils, int om,
-                           bool conj, bool acc, float scale, float2* out, hipStream_t s) {
-  if (om == kOutCoilMajor) launch_csr_om<LPR, kOutCoilMajor>(d, in, coils, conj, acc, scale, out, s);
-  else launch_csr_om<LPR, kOutRowMajor>(d, in, coils, conj, acc, scale, out, s);
+static void launch_csr_lpr(const CsrArgs& a, int om, bool conj, bool acc, hipStream_t s) {
+  if (om == kOutCoilMajor) launch_csr_om<LPR, kOutCoilMajor>(a, conj, acc, s);
+  else launch_csr_om<LPR, kOutRowMajor>(a, conj, acc, s);
 }
 
 int launch_interp_csr(const tsn_direction* d, const float2* in, uint32_t coils, int out_mode,
-                      bool conj, bool accumulate, float scale, float2* out, hipStream_t stream) {
+                      bool conj, bool accumulate, float scale, float2* out, hipStream_t stream,
+                      const ChunkScratch* scratch) {
   if (!d || !d->csr) return fail(TSN_ERR_ARGUMENT, "FP32 interpolation needs a CSR direction");
   if (coils == 0 || coils > static_cast<uint32_t>(kMaxCoils))
     return fail(TSN_ERR_ARGUMENT, "coils must be in 1..16 (got %u)", coils);
   if (!in || !out) return fail(TSN_ERR_ARGUMENT, "null input/output");
   if (out_mode != kOutCoilMajor && out_mode != kOutRowMajor)
     return fail(TSN_ERR_ARGUMENT, "unsupported FP32 output layout %d", out_mode);
-  if (coils <= 4) launch_csr_lpr<4>(d, in, coils, out_mode, conj, accumulate, scale, out, stream);
-  else if (coils <= 8) launch_csr_lpr<8>(d, in, coils, out_mode, conj, accumulate, scale, out, stream);
-  else launch_csr_lpr<16>(d, in, coils, out_mode, conj, accumulate, scale, out, stream);
+  // gathers address input rows with 32-bit byte offsets
+  if (static_cast<uint64_t>(d->h.n_cols) * coils * 8 >= (uint64_t{1} << 32))
+    return fail(TSN_ERR_ARGUMENT, "FP32 input of %u rows x %u coils exceeds 4 GiB", d->h.n_cols, coils);
+  CsrArgs a{};
+  a.row_ptr = d->row_ptr;
+  a.row_ids = d->row_ids;
+  a.cols = d->cols;
+  a.vals = d->vals;
+  a.items = d->csr_items;
+  a.n_items = d->n_csr_items;
+  a.partials = reinterpret_cast<float2*>(scratch ? scratch->partials : d->partials);
+  a.counters = scratch ? scratch->counters : d->counters;
+  if (d->n_slots && (!a.partials || !a.counters)) return fail(TSN_ERR_ARGUMENT, "missing chunk scratch");
+  a.in = in;
+  a.n_cols = d->h.n_cols;
+  a.coils = coils;
+  a.n_out = d->h.output_size;
+  a.scale = scale;
+  a.out = out;
+  if (a.n_items == 0) return TSN_OK;
+  if (coils <= 4) launch_csr_lpr<4>(a, out_mode, conj, accumulate, stream);
+  else if (coils <= 8) launch_csr_lpr<8>(a, out_mode, conj, accumulate, stream);
+  else launch_csr_lpr<16>(a, out_mode, conj, accumulate, stream);
   TSN_HIP(hipGetLastError());
   return TSN_OK;
 }
@@ -1036,7 +1158,7 @@ static void release(tsn_direction* d) {
   if (!d) return;
   void* ptrs[] = {d->group_offsets, d->group_rows, d->tile_cols, d->tile_a, d->tile_idx,
                   d->tile_dense, d->items, d->order, d->chunks, d->partials, d->counters, d->row_ptr, d->row_ids,
-                  d->cols, d->vals, d->short_rows, d->long_rows};
+                  d->cols, d->vals, d->csr_items};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (void* p : d->retired) (void)hipFree(p);
@@ -1159,6 +1281,42 @@ static int schedule_units(tsn_direction* d, uint32_t unit_tiles) {
 }
 
 }  // namespace tsn
+
+// Work items of a CSR direction in row order: runs of consecutive rows of at
+// most kCsrShortEntries entries (at most kCsrRowsPerItem rows each), and each
+// longer row cut into ceil(entries / kCsrChunkEntries) chunks of equal size
+// (within one entry); a row of several chunks takes that many partial slots
+// and one arrival counter.  Every CSR row has at least one entry.
+std::vector<tsn::CsrItem> tsn::csr_items(const std::vector<uint32_t>& ptr, uint32_t* slots, uint32_t* counters) {
+  std::vector<CsrItem> items;
+  *slots = 0;
+  *counters = 0;
+  const uint32_t rows = static_cast<uint32_t>(ptr.size()) - 1;
+  uint32_t t = 0;
+  while (t < rows) {
+    const uint32_t len = ptr[t + 1] - ptr[t];
+    if (len > static_cast<uint32_t>(kCsrShortEntries)) {
+      const uint32_t n = div_up(len, kCsrChunkEntries);
+      const uint32_t slot0 = n > 1 ? *slots : 0, counter = n > 1 ? *counters : 0;
+      if (n > 1) {
+        *slots += n;
+        *counters += 1;
+      }
+      for (uint32_t c = 0; c < n; ++c)
+        items.push_back(CsrItem{t, t + 1, ptr[t] + static_cast<uint32_t>(static_cast<uint64_t>(len) * c / n),
+                                ptr[t] + static_cast<uint32_t>(static_cast<uint64_t>(len) * (c + 1) / n), c, n,
+                                slot0, counter});
+      ++t;
+      continue;
+    }
+    const uint32_t start = t;
+    while (t < rows && t - start < static_cast<uint32_t>(kCsrRowsPerItem) &&
+           ptr[t + 1] - ptr[t] <= static_cast<uint32_t>(kCsrShortEntries))
+      ++t;
+    items.push_back(CsrItem{start, t, ptr[start], ptr[t], 0, 0, 0, 0});
+  }
+  return items;
+}
 
 std::vector<tsn::ChunkItem> tsn::long_chunks(const std::vector<uint32_t>& off, uint32_t problem, uint32_t slot_base,
                                              uint32_t counter_base, uint32_t* slots, uint32_t* counters) {
@@ -1322,10 +1480,13 @@ static int upload_pack(const HostPack& p, tsn_direction** out) {
   return TSN_OK;
 }
 
-constexpr uint32_t kMemberGroup = 8;
+constexpr uint32_t kMemberGroup = 8;  // default member group (batch autotuner candidate)
 
-int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* out) {
+int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* out, uint32_t member_group,
+                       uint32_t coil_waves) {
   *out = InterpBatch{};
+  if (member_group == 0) member_group = kMemberGroup;
+  if (coil_waves != 1 && coil_waves != 2) return fail(TSN_ERR_ARGUMENT, "coil_waves must be 1 or 2");
   if (parts.empty()) return fail(TSN_ERR_ARGUMENT, "empty interpolation batch");
   const int values = parts[0].dir->values;
   const uint32_t blocks = parts[0].blocks;
@@ -1381,10 +1542,10 @@ int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* o
     for (uint32_t x = 0; x < 8; ++x) {
       const uint32_t lo = xcd_remap(x, nd), hi = x == 7 ? nd : xcd_remap(x + 1, nd);
       const uint32_t nm = static_cast<uint32_t>(members.size());
-      for (uint32_t k0 = 0; k0 < nm; k0 += kMemberGroup)
+      for (uint32_t k0 = 0; k0 < nm; k0 += member_group)
       for (uint32_t i = lo; i < hi; ++i) {
         const uint32_t e = d->host_order[i];
-        for (uint32_t k = k0; k < std::min(nm, k0 + kMemberGroup); ++k) {
+        for (uint32_t k = k0; k < std::min(nm, k0 + member_group); ++k) {
           if (e & kChunkTag) {
             ChunkItem c = dc[e & ~kChunkTag];
             c.slot0 += slots + k * ns;
@@ -1409,6 +1570,8 @@ int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* o
   for (const std::vector<uint32_t>& v : xcd_entries) order.insert(order.end(), v.begin(), v.end());
   InterpBatch b;
   b.blocks = blocks;
+  b.coil_waves = coil_waves;
+  b.member_group = member_group;
   b.values = values;
   b.dense = std::all_of(parts.begin(), parts.end(), [](const InterpBatchPart& p) { return p.dir->tile_dense; });
   b.n_problems = static_cast<uint32_t>(probs.size());
@@ -1446,16 +1609,18 @@ void interp_batch_free(InterpBatch* b) {
   *b = InterpBatch{};
 }
 
-template <int KIND, int NB>
+template <int KIND, int NB, int WB = 1>
 static void launch_batch_nb(const InterpArgs& a, int out_mode, bool conj, hipStream_t s) {
-  if (out_mode == kOutRowMajor) launch_tiles_om<KIND, NB, true, kOutRowMajor>(a, conj, false, s);
-  else if (KIND == kSparseSplitBF16) launch_tiles_om<KIND, NB, true, kOutSplitBF16IL>(a, conj, false, s);
-  else launch_tiles_om<KIND, NB, true, kOutSplit16IL>(a, conj, false, s);
+  if (out_mode == kOutRowMajor) launch_tiles_om<KIND, NB, true, kOutRowMajor, WB>(a, conj, false, s);
+  else if (KIND == kSparseSplitBF16) launch_tiles_om<KIND, NB, true, kOutSplitBF16IL, WB>(a, conj, false, s);
+  else launch_tiles_om<KIND, NB, true, kOutSplit16IL, WB>(a, conj, false, s);
 }
 
 template <int KIND>
-static void launch_batch_kind(const InterpArgs& a, uint32_t blocks, int out_mode, bool conj, hipStream_t s) {
-  if (blocks == 2) launch_batch_nb<KIND, 2>(a, out_mode, conj, s);
+static void launch_batch_kind(const InterpArgs& a, uint32_t blocks, uint32_t coil_waves, int out_mode, bool conj,
+                              hipStream_t s) {
+  if (blocks == 2 && coil_waves == 2) launch_batch_nb<KIND, 1, 2>(a, out_mode, conj, s);
+  else if (blocks == 2) launch_batch_nb<KIND, 2>(a, out_mode, conj, s);
   else launch_batch_nb<KIND, 1>(a, out_mode, conj, s);
 }
 
@@ -1479,10 +1644,12 @@ int launch_interp_batch(const InterpBatch& b, int kind, int out_mode, bool conj,
   a.item_problem = b.item_problem;
   a.coils = kCoilsPerBlock;  // replaced per problem
   switch (kind) {
-    case kSparseSplit: launch_batch_kind<kSparseSplit>(a, b.blocks, out_mode, conj, s); break;
-    case kDenseSplit: launch_batch_kind<kDenseSplit>(a, b.blocks, out_mode, conj, s); break;
-    case kSparseHi: launch_batch_kind<kSparseHi>(a, b.blocks, out_mode, conj, s); break;
-    case kSparseSplitBF16: launch_batch_kind<kSparseSplitBF16>(a, b.blocks, out_mode, conj, s); break;
+    case kSparseSplit: launch_batch_kind<kSparseSplit>(a, b.blocks, b.coil_waves, out_mode, conj, s); break;
+    case kDenseSplit: launch_batch_kind<kDenseSplit>(a, b.blocks, b.coil_waves, out_mode, conj, s); break;
+    case kSparseHi: launch_batch_kind<kSparseHi>(a, b.blocks, b.coil_waves, out_mode, conj, s); break;
+    case kSparseSplitBF16:
+      launch_batch_kind<kSparseSplitBF16>(a, b.blocks, b.coil_waves, out_mode, conj, s);
+      break;
     default: return fail(TSN_ERR_ARGUMENT, "unknown interpolation kind %d", kind);
   }
   TSN_HIP(hipGetLastError());
@@ -1510,6 +1677,15 @@ int tsn_direction_schedule(tsn_direction* d, uint32_t unit_tiles) {
 
 uint32_t tsn_direction_unit_tiles(const tsn_direction* d) { return d ? d->unit_tiles : 0; }
 
+int tsn_direction_set_coil_waves(tsn_direction* d, uint32_t waves) {
+  if (!d) return fail(TSN_ERR_ARGUMENT, "null direction");
+  if (waves != 1 && waves != 2) return fail(TSN_ERR_ARGUMENT, "coil waves must be 1 or 2 (got %u)", waves);
+  d->coil_waves = waves;
+  return TSN_OK;
+}
+
+uint32_t tsn_direction_coil_waves(const tsn_direction* d) { return d ? d->coil_waves : 0; }
+
 int tsn_direction_autotune(tsn_direction* d, uint32_t coils, void* stream, uint32_t* best_unit_tiles,
                            float* best_us) {
   if (!d) return fail(TSN_ERR_ARGUMENT, "null direction");
@@ -1534,10 +1710,15 @@ int tsn_direction_autotune(tsn_direction* d, uint32_t coils, void* stream, uint3
   hip(hipEventCreate(&e1), "event");
   const int kind = d->values == TSN_VALUES_BF16 ? kSparseSplitBF16 : kSparseSplit;
   const uint32_t candidates[] = {2, 3, 4, 5, 6, 8, 10, 12, 14, 16};
-  uint32_t best = 0;
+  // 9..16 coils: one wave for both coil blocks, or one wave per block
+  std::vector<uint32_t> waves = {d->coil_waves};
+  if (coils > static_cast<uint32_t>(kCoilsPerBlock)) waves = {1u, 2u};
+  uint32_t best = 0, best_waves = d->coil_waves;
   float best_ms = 0.0f;
+  for (uint32_t cw : waves)
   for (uint32_t u : candidates) {
     if (rc != TSN_OK) break;
+    d->coil_waves = cw;
     rc = schedule_units(d, u);
     auto launch = [&]() {
       return launch_interp(d, kind, panel, nullptr, coils, kOutCoilMajor, false, false, 1.0f, out, nullptr,
@@ -1555,6 +1736,7 @@ int tsn_direction_autotune(tsn_direction* d, uint32_t coils, void* stream, uint3
     ms /= kReps;
     if (rc == TSN_OK && (best == 0 || ms < best_ms)) {
       best = u;
+      best_waves = cw;
       best_ms = ms;
     }
   }
@@ -1563,6 +1745,7 @@ int tsn_direction_autotune(tsn_direction* d, uint32_t coils, void* stream, uint3
   if (e1) (void)hipEventDestroy(e1);
   if (panel) (void)hipFree(panel);
   if (out) (void)hipFree(out);
+  d->coil_waves = best_waves;
   if (rc == TSN_OK) rc = schedule_units(d, best);
   if (rc != TSN_OK) return rc;
   if (best_unit_tiles) *best_unit_tiles = best;
@@ -1634,25 +1817,29 @@ int tsn_direction_upload_csr(const tsn_kb_matrix* kb, int view, tsn_direction** 
     h.n_cols = m;
     h.active_rows = static_cast<uint32_t>(rows.size());
   }
-  std::vector<int32_t> short_rows, long_rows;
-  for (uint32_t t = 0; t + 1 < ptr.size(); ++t)
-    (ptr[t + 1] - ptr[t] <= static_cast<uint32_t>(kLongRowEntries) ? short_rows : long_rows)
-        .push_back(static_cast<int32_t>(t));
-  while (long_rows.size() % 8) long_rows.push_back(-1);
+  uint32_t slots = 0, counters = 0;
+  const std::vector<CsrItem> items = csr_items(ptr, &slots, &counters);
+  if (slots > kMaxCsrSlots) return fail(TSN_ERR_ARGUMENT, "%u long-row chunks (at most %u)", slots, kMaxCsrSlots);
   auto* d = new tsn_direction();
   d->h = h;
   d->csr = true;
-  d->n_short = static_cast<uint32_t>(short_rows.size());
-  d->n_long_rows = static_cast<uint32_t>(long_rows.size());
   (void)hipGetDevice(&d->device);
   d->csr_rows = static_cast<uint32_t>(rows.size());
+  d->n_csr_items = static_cast<uint32_t>(items.size());
+  d->n_slots = slots;
+  d->n_counters = counters;
   size_t bytes = 0;
   int rc = device_copy(ptr.data(), ptr.size(), &d->row_ptr, &bytes);
   if (rc == TSN_OK) rc = device_copy(rows.data(), rows.size(), &d->row_ids, &bytes);
   if (rc == TSN_OK) rc = device_copy(cols.data(), cols.size(), &d->cols, &bytes);
   if (rc == TSN_OK) rc = device_copy(vals.data(), vals.size(), &d->vals, &bytes);
-  if (rc == TSN_OK) rc = device_copy(short_rows.data(), short_rows.size(), &d->short_rows, &bytes);
-  if (rc == TSN_OK) rc = device_copy(long_rows.data(), long_rows.size(), &d->long_rows, &bytes);
+  if (rc == TSN_OK) rc = device_copy(items.data(), items.size(), &d->csr_items, &bytes);
+  if (rc == TSN_OK && slots) {
+    const std::vector<float> zeros(static_cast<size_t>(slots) * 2 * kMaxCoils, 0.0f);
+    const std::vector<uint32_t> zero_counters(round_up(counters, 4u), 0u);
+    rc = device_copy(zeros.data(), zeros.size(), &d->partials, &bytes);
+    if (rc == TSN_OK) rc = device_copy(zero_counters.data(), zero_counters.size(), &d->counters, &bytes);
+  }
   if (rc != TSN_OK) {
     release(d);
     return rc;

@@ -24,8 +24,6 @@
 namespace tsn {
 int launch_split_panel_il(const float2* in, uint32_t rows, uint32_t coils, float scale, uint32_t* out,
                           hipStream_t stream, int values);
-int launch_interp_csr(const tsn_direction* d, const float2* in, uint32_t coils, int out_mode,
-                      bool conj, bool accumulate, float scale, float2* out, hipStream_t stream);
 int launch_split_panel(const float2* in, uint32_t rows, uint32_t coils, float scale, _Float16* hi,
                        _Float16* res, hipStream_t stream);
 }  // namespace tsn
@@ -129,8 +127,10 @@ int op_forward_samples(tsn_plan* pl, float2* out_coil_major, hipStream_t s) {
   const float scale = grid_norm(pl);
   if (is_fp32(pl)) {
     if (out_coil_major)
-      return launch_interp_csr(pl->fwd, pl->pan32, pl->cfg.coils, 0, false, false, scale, out_coil_major, s);
-    return launch_interp_csr(pl->fwd, pl->pan32, pl->cfg.coils, 1, false, false, scale, pl->smp32, s);
+      return launch_interp_csr(pl->fwd, pl->pan32, pl->cfg.coils, 0, false, false, scale, out_coil_major, s,
+                               &pl->fwd_chunks);
+    return launch_interp_csr(pl->fwd, pl->pan32, pl->cfg.coils, 1, false, false, scale, pl->smp32, s,
+                             &pl->fwd_chunks);
   }
   if (out_coil_major)
     return launch_interp(pl->fwd, interp_kind(pl), pl->pan_il, nullptr, pl->cfg.coils, 0, false, false, scale,
@@ -143,7 +143,8 @@ int op_forward_samples(tsn_plan* pl, float2* out_coil_major, hipStream_t s) {
 // sample panels -> grid (conjugate transpose interpolation)
 int op_adjoint_grid(tsn_plan* pl, hipStream_t s) {
   if (is_fp32(pl))
-    return launch_interp_csr(pl->adj, pl->smp32, pl->cfg.coils, 1, true, false, 1.0f, pl->grid, s);
+    return launch_interp_csr(pl->adj, pl->smp32, pl->cfg.coils, 1, true, false, 1.0f, pl->grid, s,
+                             &pl->adj_chunks);
   return launch_interp(pl->adj, interp_kind(pl), pl->smp_il, nullptr, pl->cfg.coils, 1, true, false, 1.0f,
                        reinterpret_cast<float*>(pl->grid), nullptr, nullptr, s, true, nullptr, &pl->adj_chunks);
 }
@@ -340,12 +341,12 @@ int tsn_plan_create(const tsn_plan_config* cfg, const tsn_direction* fwd, const 
   }
   for (float2** v : {&pl->x, &pl->r, &pl->r2, &pl->p, &pl->p2, &pl->ap, &pl->tmp}) A(v, px);
   A(&pl->partial, 3 * static_cast<size_t>(c.image1));
-  if (!fp32) {
-    A(&pl->fwd_chunks.partials, static_cast<size_t>(fwd->n_slots) * kSlotFloats);
-    A(&pl->fwd_chunks.counters, round_up(fwd->n_counters, 4u));
-    A(&pl->adj_chunks.partials, static_cast<size_t>(adj->n_slots) * kSlotFloats);
-    A(&pl->adj_chunks.counters, round_up(adj->n_counters, 4u));
-  }
+  // slots: kSlotFloats floats per packed chunk, kMaxCoils float2 per CSR chunk
+  const size_t slot_floats = fp32 ? 2 * kMaxCoils : kSlotFloats;
+  A(&pl->fwd_chunks.partials, static_cast<size_t>(fwd->n_slots) * slot_floats);
+  A(&pl->fwd_chunks.counters, round_up(fwd->n_counters, 4u));
+  A(&pl->adj_chunks.partials, static_cast<size_t>(adj->n_slots) * slot_floats);
+  A(&pl->adj_chunks.counters, round_up(adj->n_counters, 4u));
   A(&pl->rho, kRhoInitial);
   pl->rho_cap = kRhoInitial;
   A(&pl->status, 1);
@@ -426,7 +427,8 @@ int tsn_data_consistency(tsn_plan* pl, const void* image, const void* measuremen
     dc.y = static_cast<const float*>(measurement);
     dc.row_stride = 2;
     dc.coil_stride = 2 * pl->m;
-    TSN_CHECK(launch_interp_csr(pl->fwd, pl->pan32, coils, 1, false, false, grid_norm(pl), pl->smp32, s));
+    TSN_CHECK(launch_interp_csr(pl->fwd, pl->pan32, coils, 1, false, false, grid_norm(pl), pl->smp32, s,
+                                &pl->fwd_chunks));
     TSN_CHECK(launch_dc_residual_rows(pl->smp32, pl->m, coils, dc, s));
   } else {
     // y to rows (one 64-byte read per sample row in the epilogue), residual fused

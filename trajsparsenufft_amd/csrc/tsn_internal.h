@@ -21,7 +21,17 @@ constexpr int kN = 16;      // channels per panel block (8 coils x Re/Im)
 constexpr int kCoilsPerBlock = 8;
 constexpr int kMaxCoils = 16;
 constexpr int kSplitAboveTiles = 16;  // tiles per unit at most; longer groups are cut into chunks of this size
-constexpr int kLongRowEntries = 128;  // FP32 CSR rows longer than this get a whole block
+// Interleaved-panel interpolation of 9..16 coils: waves per work entry, 1 (one
+// wave computes both coil blocks) or 2 (one wave per block, same entry).
+constexpr uint32_t kDefaultCoilWaves = 1;
+// FP32 CSR work items: runs of at most kCsrRowsPerItem consecutive rows of at
+// most kCsrShortEntries entries each; longer rows are cut into chunks of at
+// most kCsrChunkEntries entries.  A wave stages an item's entries in LDS.
+constexpr int kCsrRowsPerItem = 8;
+constexpr int kCsrShortEntries = 64;
+constexpr int kCsrChunkEntries = 256;
+constexpr int kCsrStageEntries = kCsrRowsPerItem * kCsrShortEntries;
+static_assert(kCsrChunkEntries <= kCsrStageEntries, "a chunk must fit the LDS stage");
 // Device column words of packed tiles hold column * 32 (a byte offset) below
 // bit 31, so a pack may have at most 2^26 columns.
 constexpr uint32_t kMaxDeviceCols = 1u << 26;
@@ -131,6 +141,16 @@ struct ChunkItem {
   uint32_t counter;  // arrival counter of the group (reset to 0 by the last arrival)
   uint32_t problem;  // batched launches: the problem table entry
 };
+// One work item of the FP32 CSR kernel: a run of short rows (n_chunks 0), or
+// chunk `chunk` of the n_chunks chunks of one long row.  A row of several
+// chunks has partial slots slot0 .. slot0 + n_chunks - 1 (kMaxCoils float2
+// each) and arrival counter `counter`.
+struct CsrItem {
+  uint32_t row_begin, row_end;  // CSR rows (a chunk: row_end = row_begin + 1)
+  uint32_t e_begin, e_end;      // the item's entries, contiguous in the CSR arrays
+  uint32_t chunk, n_chunks;
+  uint32_t slot0, counter;
+};
 // Floats of one partial slot: two coil blocks x (re, im) x 64 lanes x float4.
 constexpr uint32_t kSlotFloats = 2 * 2 * 64 * 4;
 // Launch order entries: a unit index, or kChunkTag | chunk index.  Units and
@@ -140,6 +160,11 @@ constexpr uint32_t kSlotFloats = 2 * 2 * 64 * 4;
 constexpr uint32_t kChunkTag = 0x80000000u;
 // Chunks of a direction's long groups with slots and counters numbered from
 // slot_base / counter_base (host side).
+// FP32 CSR work items of row pointers ptr (host side); slots and counters of
+// the rows of several chunks are numbered from 0.
+std::vector<CsrItem> csr_items(const std::vector<uint32_t>& ptr, uint32_t* slots, uint32_t* counters);
+// Slot offsets of CSR chunks are 32-bit buffer offsets of kMaxCoils float2.
+constexpr uint32_t kMaxCsrSlots = (1u << 31) / (kMaxCoils * 8);
 std::vector<ChunkItem> long_chunks(const std::vector<uint32_t>& group_offsets, uint32_t problem, uint32_t slot_base,
                                    uint32_t counter_base, uint32_t* slots, uint32_t* counters);
 // One part of a batched interpolation launch: one or two coil blocks of one
@@ -167,10 +192,15 @@ struct InterpBatch {
   uint32_t* order = nullptr;     // launch order of units and chunks (kChunkTag marks a chunk)
   uint32_t n_problems = 0, n_items = 0, n_chunks = 0;
   uint32_t blocks = 1;  // coil blocks per part (every part of a batch has the same)
+  uint32_t coil_waves = kDefaultCoilWaves;  // parts of two blocks: 1 wave for both, or 2 (one per block)
+  uint32_t member_group = 0;                // parts sharing a direction that follow each entry
   int values = TSN_VALUES_FP16;
   bool dense = false;
 };
-int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* out);
+// member_group: parts sharing a direction that follow each of its entries
+// (0 = the default, 8); coil_waves: waves per entry of two-block parts.
+int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* out, uint32_t member_group = 0,
+                       uint32_t coil_waves = kDefaultCoilWaves);
 void interp_batch_free(InterpBatch* b);
 // kind: 0 split FP16, 1 dense control, 2 FP16 hi only, 3 split BF16; out_mode:
 // 1 row-major float2, 3 interleaved split (FP16 or BF16 as the kind).
@@ -185,6 +215,10 @@ struct ChunkScratch {
   float* partials = nullptr;
   uint32_t* counters = nullptr;
 };
+// FP32 CSR interpolation; scratch null: the direction's own chunk scratch.
+int launch_interp_csr(const tsn_direction* d, const float2* in, uint32_t coils, int out_mode, bool conj,
+                      bool accumulate, float scale, float2* out, hipStream_t stream,
+                      const ChunkScratch* scratch = nullptr);
 // Packed interpolation (kind and out_mode as below); scratch null: the
 // direction's own.
 int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* res, uint32_t coils,
@@ -244,6 +278,7 @@ struct tsn_direction {
   uint32_t n_items = 0, n_chunks = 0;
   uint32_t n_slots = 0, n_counters = 0;  // sizes of the chunk scratch (ChunkScratch)
   uint32_t unit_tiles = 0;           // tile budget of one regular unit
+  uint32_t coil_waves = tsn::kDefaultCoilWaves;  // interleaved launches of 9..16 coils: waves per entry (1 or 2)
   std::vector<uint32_t> host_offsets;  // group_offsets kept for rescheduling
   uint32_t* order = nullptr;         // [n_items + n_chunks] launch order (kChunkTag marks chunks)
   std::vector<tsn::WorkItem> host_items;  // the current schedule (batched launches merge it)
@@ -255,8 +290,9 @@ struct tsn_direction {
   int32_t* row_ids = nullptr;    // [csr_rows] logical output row
   int32_t* cols = nullptr;       // [nnz]
   float2* vals = nullptr;        // [nnz]
-  int32_t* short_rows = nullptr; // CSR rows with <= kLongRowEntries entries
-  int32_t* long_rows = nullptr;  // longer rows, padded with -1 to a multiple of 8
-  uint32_t n_short = 0, n_long_rows = 0;
+  tsn::CsrItem* csr_items = nullptr;  // [n_csr_items] work items in launch order
+  uint32_t n_csr_items = 0;
+  // (partials / counters / n_slots / n_counters above hold the long-row chunk
+  // scratch of a CSR direction: kMaxCoils float2 per slot)
   size_t bytes = 0;
 };

@@ -99,6 +99,16 @@ class Direction:
         check(lib().tsn_direction_autotune(self._h, int(coils), _stream(), C.byref(best), C.byref(us)))
         return int(best.value), float(us.value)
 
+    @property
+    def coil_waves(self) -> int:
+        """Waves per work entry of interleaved-panel launches of 9..16 coils
+        (1: one wave for both coil blocks, 2: one per block)."""
+        return int(lib().tsn_direction_coil_waves(self._h))
+
+    @coil_waves.setter
+    def coil_waves(self, waves: int) -> None:
+        check(lib().tsn_direction_set_coil_waves(self._h, int(waves)))
+
     def schedule(self, unit_tiles: int = 0) -> None:
         """Re-cut the work units (runs of whole groups of at most unit_tiles
         tiles; 0 = default).  Results are bitwise independent of the cut."""

@@ -195,6 +195,7 @@ class Batch:
         self.grid_size = tuple(grid_size)
         self.precision = precision
         self.count = len(problems)
+        self.tuning = None  # the autotuner's choice (build_batch(autotune=True))
         arr = (BatchProblem * self.count)()
         self._keep = []
         self.coils, self.samples = [], []
@@ -247,6 +248,15 @@ class Batch:
         return (outs, status, ptrs(measurements), (C.c_float * self.count)(*map(float, scales)),
                 ptrs([o.image for o in outs]), ptrs([o.residuals for o in outs]))
 
+    def autotune(self) -> dict:
+        """Time the batched interpolation steps over the member group (frames
+        sharing a pack per work entry) and the waves per entry of 9..16-coil
+        problems, and keep the fastest (results unchanged bit for bit)."""
+
+        group, waves, us = C.c_uint32(), C.c_uint32(), C.c_float()
+        check(lib().tsn_batch_autotune(self._h, _stream(), C.byref(group), C.byref(waves), C.byref(us)))
+        return {"member_group": int(group.value), "coil_waves": int(waves.value), "us": float(us.value)}
+
     def cg(self, measurements, iterations: int = 10, operator_scales=None, lam: float = 1e-4, outs=None):
         """Enqueue all solves on the current stream; returns one CgResult per problem."""
 
@@ -273,7 +283,8 @@ class Batch:
 def build_batch(cases, precision: str = "split16", shared: bool = False, autotune: bool = False) -> Batch:
     """cases: sequence of (TrajectoryPacks, smaps tensor).  With ``shared`` the
     directions of the first case serve every problem (one packed trajectory
-    for a frame series); ``autotune`` tunes each direction's schedule first."""
+    for a frame series); ``autotune`` tunes each direction's schedule, then
+    the batch's member group and coil waves."""
 
     values = "bf16" if precision == "splitbf16" else "fp16"
     problems, dirs = [], {}
@@ -288,7 +299,10 @@ def build_batch(cases, precision: str = "split16", shared: bool = False, autotun
         problems.append((*dirs[key], smaps))
     packs0, smaps0 = cases[0]
     apod = torch.from_numpy(np.ascontiguousarray(packs0.apod)).to(smaps0.device)
-    return Batch(problems, apod, packs0.kb.image_size, packs0.kb.grid_size, precision)
+    batch = Batch(problems, apod, packs0.kb.image_size, packs0.kb.grid_size, precision)
+    if autotune:
+        batch.tuning = batch.autotune()
+    return batch
 
 
 def check_cg(result: CgResult) -> None:
