@@ -472,6 +472,13 @@ __device__ __forceinline__ void stream_tiles(const InterpArgs& a, const TileRsrc
 // agent-scope fence here would write back the whole L2 per chunk (measured:
 // the launch went from 29 to 230 us).  The last wave resets the counter for
 // the next launch.  No wave waits on another.
+// The protocol rests on gfx950's mapping of device-scope accesses (sc1
+// stores and loads reach the coherent level past the per-XCD L2, in issue
+// order with respect to the counter's atomic after the store wait) rather
+// than on language-level acquire/release; other targets are refused here.
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(__gfx950__)
+#error "the chunk-slot protocol of the interpolation kernels is written for gfx950"
+#endif
 constexpr int kCpolDevice = 16;  // cache policy bits of a buffer access: sc1 (device scope)
 constexpr uint32_t kSlotBatch = 8;  // slots loaded before the first add (one round trip for <= 8 chunks)
 // Slot byte offsets are 32-bit buffer offsets: at most 2^31 bytes of slots.
