@@ -193,6 +193,20 @@ void tsn_direction_free(tsn_direction* dir);
 int tsn_interp(const tsn_direction* dir, const void* panel_hi, const void* panel_res, void* out,
                uint32_t coils, int conjugate, int accumulate, float scale, int out_layout,
                void* stream);
+/* tsn_interp without any allocation or direction-held scratch: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The long-group chunk partials and arrival counters live
+ * in one static device array per GPU (6 MiB + 16 KiB, held until the process
+ * ends): at most 1536 chunks (4 KiB each) and 4096 long groups per direction,
+ * else TSN_ERR_ARGUMENT.  The direction must live on the current device.  The
+ * counters are cleared by a fill kernel on `stream` at each call.  Every call
+ * on one GPU shares that array, so calls that overlap on one GPU (other
+ * streams, other threads) race; order them.  Same arguments, kernels and
+ * results (bitwise) as tsn_interp. */
+int tsn_interp_noalloc(const tsn_direction* dir, const void* panel_hi, const void* panel_res, void* out,
+                       uint32_t coils, int conjugate, int accumulate, float scale, int out_layout,
+                       void* stream);
 /* Same operator on the dense-MFMA control (tiles expanded to 16x32 on first use). */
 int tsn_interp_dense(const tsn_direction* dir, const void* panel_hi, const void* panel_res,
                      void* out, uint32_t coils, int conjugate, int accumulate, float scale,

@@ -78,6 +78,7 @@ SIGNATURES = {
     "tsn_direction_autotune": (C.c_int, [P, C.c_uint32, P, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
     "tsn_direction_free": (None, [P]),
     "tsn_interp": (C.c_int, [P, P, P, P, U32, C.c_int, C.c_int, C.c_float, C.c_int, P]),
+    "tsn_interp_noalloc": (C.c_int, [P, P, P, P, U32, C.c_int, C.c_int, C.c_float, C.c_int, P]),
     "tsn_interp_dense": (C.c_int, [P, P, P, P, U32, C.c_int, C.c_int, C.c_float, C.c_int, P]),
     "tsn_interp_f32": (C.c_int, [P, P, P, U32, C.c_int, C.c_int, C.c_float, C.c_int, P]),
     "tsn_split_panel": (C.c_int, [P, U32, U32, C.c_float, P, P, P]),

@@ -15,6 +15,7 @@
 // over several waves and reduced deterministically; block -> item mapping
 // keeps Morton-adjacent groups on one XCD (shared L2).
 #include <algorithm>
+#include <mutex>
 #include <numeric>
 
 #include "tsn_internal.h"
@@ -1663,9 +1664,80 @@ int launch_interp_batch(const InterpBatch& b, int kind, int out_mode, bool conj,
   return TSN_OK;
 }
 
+// ------------------------------------------------- no-allocation entry point
+// tsn_interp_noalloc takes the long-group chunk scratch from this file-scope
+// array instead of the direction's: kStaticSlots partial slots (4 KiB each)
+// followed by kStaticCounters arrival counters, 6 MiB + 16 KiB in all.  Every
+// device the library runs on holds its own copy until the process ends.  On
+// the host `scratch` names a host-side stand-in, never the device memory: the
+// device address comes from hipGetSymbolAddress, once per device, and only
+// that address is handed to kernels.
+constexpr uint32_t kStaticSlots = 1536;
+constexpr uint32_t kStaticCounters = 4096;
+constexpr size_t kStaticScratchWords = static_cast<size_t>(kStaticSlots) * kSlotFloats + kStaticCounters;
+static __device__ __align__(256) uint32_t scratch[kStaticScratchWords];
+constexpr int kMaxStaticDevices = 64;
+
+// Device address of `scratch` on the current device (looked up on the first
+// call per device).
+static int static_scratch(uint32_t** out) {
+  static std::mutex lock;
+  static uint32_t* address[kMaxStaticDevices] = {};
+  int device = 0;
+  TSN_HIP(hipGetDevice(&device));
+  if (device < 0 || device >= kMaxStaticDevices)
+    return fail(TSN_ERR_ARGUMENT, "device %d is beyond the %d devices of the static scratch table", device,
+                kMaxStaticDevices);
+  std::lock_guard<std::mutex> guard(lock);
+  if (!address[device]) {
+    void* ptr = nullptr;
+    TSN_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(TSN_ERR_RUNTIME, "no device address for the static scratch on device %d", device);
+    address[device] = static_cast<uint32_t*>(ptr);
+  }
+  *out = address[device];
+  return TSN_OK;
+}
+
+__global__ __launch_bounds__(256) void clear_words_kernel(uint4* __restrict__ p, uint32_t n4) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n4) p[e] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 }  // namespace tsn
 
 extern "C" {
+
+int tsn_interp_noalloc(const tsn_direction* d, const void* hi, const void* res, void* out, uint32_t coils,
+                       int conjugate, int accumulate, float scale, int out_layout, void* stream) {
+  if (out_layout != TSN_OUT_COIL_MAJOR && out_layout != TSN_OUT_ROW_MAJOR)
+    return fail(TSN_ERR_ARGUMENT, "unknown output layout %d", out_layout);
+  if (!d || d->csr) return fail(TSN_ERR_ARGUMENT, "interpolation needs a packed (non-CSR) direction");
+  if (d->n_slots > kStaticSlots || d->n_counters > kStaticCounters)
+    return fail(TSN_ERR_ARGUMENT,
+                "direction needs %u chunk slots and %u counters; the static scratch holds %u and %u",
+                d->n_slots, d->n_counters, kStaticSlots, kStaticCounters);
+  int device = 0;
+  TSN_HIP(hipGetDevice(&device));
+  if (device != d->device)
+    return fail(TSN_ERR_ARGUMENT, "direction lives on device %d, the current device is %d", d->device, device);
+  ChunkScratch chunk;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (d->n_chunks) {
+    uint32_t* base = nullptr;
+    TSN_CHECK(static_scratch(&base));
+    chunk.partials = reinterpret_cast<float*>(base);
+    chunk.counters = base + static_cast<size_t>(kStaticSlots) * kSlotFloats;
+    // the counters this launch uses start at zero whatever the last call left
+    const uint32_t n4 = div_up(d->n_counters, 4);
+    hipLaunchKernelGGL(clear_words_kernel, dim3(div_up(n4, 256)), dim3(256), 0, s,
+                       reinterpret_cast<uint4*>(chunk.counters), n4);
+    TSN_HIP(hipGetLastError());
+  }
+  return launch_interp(d, res ? kSparseSplit : kSparseHi, hi, res, coils, out_layout, conjugate != 0,
+                       accumulate != 0, scale, static_cast<float*>(out), nullptr, nullptr, s, false, nullptr,
+                       d->n_chunks ? &chunk : nullptr);
+}
 
 int tsn_direction_upload(const tsn_host_pack* pack, tsn_direction** out) {
   if (!pack || !out) return fail(TSN_ERR_ARGUMENT, "null argument");
