@@ -363,6 +363,23 @@ def interp_roofline(torch, packs, steps, warmup, log):
                      "traffic": measured_traffic(kernel)}
         log(f"[bench] interp {name} {ms * 1e3:.1f} us ({res[name]['GBs']:.0f} GB/s algorithmic, "
             f"HBM traffic {res[name]['traffic']})")
+    # the plain FP32 path's CSR launches as its reconstruction runs them
+    # (row-major float2 rows in and out): B = nnz * 12 + n_in * C * 8 + n_out * C * 8
+    cf, ca = Direction.csr(packs.kb, "G"), Direction.csr(packs.kb, "G_T")
+    xg = torch.from_numpy((rng.standard_normal((g, COILS)) + 1j * rng.standard_normal((g, COILS))).astype(np.complex64)).cuda()
+    xs = torch.from_numpy((rng.standard_normal((m, COILS)) + 1j * rng.standard_normal((m, COILS))).astype(np.complex64)).cuda()
+    os_ = torch.empty((m, COILS), dtype=torch.complex64, device="cuda")
+    og = torch.zeros((g, COILS), dtype=torch.complex64, device="cuda")
+    f32_ms = time_launches(torch, lambda: device.interp_f32(cf, xg, os_, COILS, False, False, 1.0 / GRID, "row_major"),
+                           steps, warmup)
+    a32_ms = time_launches(torch, lambda: device.interp_f32(ca, xs, og, COILS, True, False, 1.0, "row_major"),
+                           steps, warmup)
+    for name, ms, n_in, n_out, conj in (("fp32_forward", f32_ms, g, m, "false"), ("fp32_adjoint", a32_ms, m, g, "true")):
+        b = nnz * 12 + n_in * COILS * 8 + n_out * COILS * 8
+        kernel = f"interp_csr_kernel<8, 1, {conj}, false>"
+        res[name] = {"kernel": kernel, "us": ms * 1e3, "bytes": b, "GBs": b / (ms * 1e-3) / 1e9,
+                     "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": measured_traffic(kernel)}
+        log(f"[bench] FP32 CSR {name[5:]} {ms * 1e3:.1f} us ({res[name]['GBs']:.0f} GB/s algorithmic)")
     res["dominant"] = res["adjoint"] if adj_ms >= fwd_ms else res["forward"]
     res["traffic_source"] = (f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --roofline-only, "
                              f"scaled by a {COPY_BYTES >> 20} MiB copy: {TRAFFIC_TABLE}")
@@ -392,7 +409,7 @@ def roofline_only(args) -> int:
     for _ in range(3):
         plan.cg(y, CG_ITERS, scale, LAMBDA)
     torch.cuda.synchronize()
-    print(json.dumps({k: roof[k] for k in ("forward", "adjoint")}), flush=True)
+    print(json.dumps({k: roof[k] for k in ("forward", "adjoint", "fp32_forward", "fp32_adjoint")}), flush=True)
     return 0
 
 
@@ -580,7 +597,7 @@ def main() -> int:
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dominant["GBs"] / HBM_PEAK_GBS,
                      "traffic": dominant["traffic"], "algorithmic_bytes": dominant["bytes"],
                      "avg_launch_us": dominant["us"], "traffic_source": roof["traffic_source"]},
-        "interp": {k: roof[k] for k in ("forward", "adjoint")} | {
+        "interp": {k: roof[k] for k in ("forward", "adjoint", "fp32_forward", "fp32_adjoint")} | {
             "pack_tiles": {"G": fh["tiles"], "G_T": ah["tiles"]}},
         "prep_seconds": prep_s,
         "cpu_baseline": cpu,

@@ -156,11 +156,14 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
                 static_cast<uint64_t>(a.n1) * a.cstride, min(8u, a.coils - chunk * 8), a.cstride, lds);
 }
 
-// The second-axis passes take kRowsPerBlock consecutive grid rows per block:
+// The second-axis passes take rows_per_block consecutive grid rows per block:
 // the next row's loads are issued as soon as the current row is in LDS, so
 // they overlap its transform and stores, and the launch has half the blocks
 // to dispatch.
-constexpr uint32_t kRowsPerBlock = 2;
+// Length 768 (R = 12) takes one row per block: the second row's registers
+// would push the kernel to 2-3 waves per SIMD.
+template <int R>
+constexpr uint32_t rows_per_block() { return R == 12 ? 1 : 2; }
 
 // Forward, second axis: grid row k0 -> panel rows k0*K1 + k1.  OUT 0: FP16
 // hi/residual interleaved N16 panel blocks; OUT 1: float2 [row][coils];
@@ -173,6 +176,7 @@ __global__ __launch_bounds__(512) void fwd_axis1_kernel(FftArgs args) {
   extern __shared__ float2 lds[];
   constexpr int K = 64 * R;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr uint32_t kRowsPerBlock = rows_per_block<R>();
   const uint32_t row0 = blockIdx.x * kRowsPerBlock, chunk = blockIdx.y;
   const uint32_t cw = min(8u, a.coils - chunk * 8);
   const uint64_t grid_rows = static_cast<uint64_t>(a.k0) * K;
@@ -242,6 +246,7 @@ __global__ __launch_bounds__(512) void inv_axis1_kernel(FftArgs args) {
   extern __shared__ float2 lds[];
   constexpr int K = 64 * R;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr uint32_t kRowsPerBlock = rows_per_block<R>();
   const uint32_t row0 = xcd_remap(blockIdx.x, gridDim.x) * kRowsPerBlock, chunk = blockIdx.y;
   const uint32_t cw = min(8u, a.coils - chunk * 8);
   auto g_row = [&](uint32_t k0) { return a.grid + static_cast<size_t>(k0) * K * a.cstride + chunk * 8; };
@@ -397,13 +402,13 @@ int fft_forward(const FftArgs& a, int out_kind, hipStream_t s, uint32_t problems
   TSN_DISPATCH_R(r0, hipLaunchKernelGGL(fwd_axis0_kernel<R>, dim3(a.n1, chunks, problems), dim3(512),
                                         lds_bytes_for<R>(), s, a));
   if (out_kind == 0) {
-    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 0>), dim3(div_up(a.k0, kRowsPerBlock), chunks, problems), dim3(512),
+    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 0>), dim3(div_up(a.k0, rows_per_block<R>()), chunks, problems), dim3(512),
                                           lds_bytes_for<R>(), s, a));
   } else if (out_kind == 2) {
-    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 2>), dim3(div_up(a.k0, kRowsPerBlock), chunks, problems), dim3(512),
+    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 2>), dim3(div_up(a.k0, rows_per_block<R>()), chunks, problems), dim3(512),
                                           lds_bytes_for<R>(), s, a));
   } else {
-    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 1>), dim3(div_up(a.k0, kRowsPerBlock), chunks, problems), dim3(512),
+    TSN_DISPATCH_R(r1, hipLaunchKernelGGL((fwd_axis1_kernel<R, 1>), dim3(div_up(a.k0, rows_per_block<R>()), chunks, problems), dim3(512),
                                           lds_bytes_for<R>(), s, a));
   }
   TSN_HIP(hipGetLastError());
@@ -413,7 +418,7 @@ int fft_forward(const FftArgs& a, int out_kind, hipStream_t s, uint32_t problems
 int fft_inverse(const FftArgs& a, hipStream_t s, uint32_t problems) {
   const int r0 = radix_of(a.k0), r1 = radix_of(a.k1);
   const uint32_t chunks = (a.cstride + 7) / 8;
-  TSN_DISPATCH_R(r1, hipLaunchKernelGGL(inv_axis1_kernel<R>, dim3(div_up(a.k0, kRowsPerBlock), chunks, problems), dim3(512),
+  TSN_DISPATCH_R(r1, hipLaunchKernelGGL(inv_axis1_kernel<R>, dim3(div_up(a.k0, rows_per_block<R>()), chunks, problems), dim3(512),
                                         lds_bytes_for<R>(), s, a));
   TSN_DISPATCH_R(r0, hipLaunchKernelGGL(inv_axis0_kernel<R>, dim3(a.n1, 1, problems), dim3(512),
                                         lds_bytes_for<R>(), s, a));

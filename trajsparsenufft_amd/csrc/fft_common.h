@@ -241,6 +241,48 @@ __device__ __forceinline__ void dft8(const float2* a, float2* X) {
   X[7] = csub(E[3], o3);
 }
 
+// 3-point DFT: X1, X2 = a0 - (a1 + a2)/2 -/+ i (sqrt(3)/2)(a1 - a2) (forward;
+// inverse conjugate).
+template <bool INV>
+__device__ __forceinline__ void dft3(float2 a0, float2 a1, float2 a2, float2& X0, float2& X1, float2& X2) {
+  constexpr float s3 = 0.86602540378443864676f;
+  const float2 t = cadd(a1, a2);
+  const float2 u = csub(a1, a2);
+  X0 = cadd(a0, t);
+  const float2 m = make_float2(a0.x - 0.5f * t.x, a0.y - 0.5f * t.y);
+  const float2 n = mul_mi<INV>(make_float2(s3 * u.x, s3 * u.y));
+  X1 = cadd(m, n);
+  X2 = csub(m, n);
+}
+
+// 12-point DFT as 3 x 4 (Cooley-Tukey): with j = j1 + 4 j2 and k = ka + 3 kb,
+// X[k] = sum_j1 W4^(j1 kb) W12^(j1 ka) sum_j2 W3^(j2 ka) a[j1 + 4 j2]: four
+// 3-point DFTs, six constant twiddles, three 4-point DFTs.
+template <bool INV>
+__device__ __forceinline__ void dft12(const float2* a, float2* X) {
+  constexpr float h = 0.5f, s3 = 0.86602540378443864676f;
+  // forward W12^e for e = 1, 2, 3, 4, 6 (inverse: their conjugates)
+  const float2 w1 = make_float2(s3, -h), w2 = make_float2(h, -s3), w3 = make_float2(0.f, -1.f),
+               w4 = make_float2(-h, -s3), w6 = make_float2(-1.f, 0.f);
+  float2 A[4][3];
+#pragma unroll
+  for (int j1 = 0; j1 < 4; ++j1) dft3<INV>(a[j1], a[j1 + 4], a[j1 + 8], A[j1][0], A[j1][1], A[j1][2]);
+  A[1][1] = twiddle<INV>(A[1][1], w1);
+  A[1][2] = twiddle<INV>(A[1][2], w2);
+  A[2][1] = twiddle<INV>(A[2][1], w2);
+  A[2][2] = twiddle<INV>(A[2][2], w4);
+  A[3][1] = twiddle<INV>(A[3][1], w3);
+  A[3][2] = twiddle<INV>(A[3][2], w6);
+#pragma unroll
+  for (int ka = 0; ka < 3; ++ka) {
+    const float2 in[4] = {A[0][ka], A[1][ka], A[2][ka], A[3][ka]};
+    float2 out[4];
+    dft4<INV, 4>(in, out);
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) X[ka + 3 * kb] = out[kb];
+  }
+}
+
 // Orders one wave's LDS accesses around this point for data that only this
 // wave writes and reads (its own slice): a wave's LDS instructions execute in
 // issue order, so keeping the compiler from moving accesses across the point
@@ -306,14 +348,8 @@ __device__ __forceinline__ void wave_fft(float2 (&x)[R], float2* buf, const floa
   } else if constexpr (R == 4) {
     dft4<INV, HALF_IN ? 2 : 4>(x, y);
   } else {
-    constexpr int JN = HALF_IN ? R / 2 : R;
-#pragma unroll
-    for (int k1 = 0; k1 < R; ++k1) {
-      float2 s = x[0];
-#pragma unroll
-      for (int j = 1; j < JN; ++j) s = cadd(s, twiddle<INV>(x[j], tw[((j * k1) % R) * 64]));
-      y[k1] = s;
-    }
+    // R = 12; with HALF_IN the caller's x[6 .. 11] are zero constants
+    dft12<INV>(x, y);
   }
 #pragma unroll
   for (int k1 = 1; k1 < R; ++k1) y[k1] = twiddle<INV>(y[k1], R <= 8 ? twp.a[k1] : tw[lane * k1]);

@@ -15,6 +15,7 @@
 // over several waves and reduced deterministically; block -> item mapping
 // keeps Morton-adjacent groups on one XCD (shared L2).
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <numeric>
 
@@ -636,14 +637,13 @@ struct CsrArgs {
 // CSR arrays; the wave copies them into LDS with coalesced loads, all issued
 // before the first wait.  LPR lanes (one coil each) then work on a row: the
 // lane groups of a run take its rows, the lane groups of a chunk stride its
-// entries and are summed in a fixed butterfly order.  Each group reads
-// kCsrBatch (column, value) pairs from LDS (one broadcast address per group),
-// issues their kCsrBatch gathers of input rows float2 [row][coils] and then
+// entries and are summed in a fixed butterfly order.  Each group reads the
+// column ids of a batch of entries from LDS (one broadcast address per group),
+// issues all of the batch's gathers of input rows float2 [row][coils] and then
 // accumulates in entry order, so a row's sum never depends on the schedule.
 // A row of several chunks publishes one partial per chunk to a slot, and the
 // wave that counts the row's last arrival adds the slots in chunk order and
 // writes the row (the slot protocol of the packed kernel's long groups).
-constexpr int kCsrBatch = 16;
 typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
 typedef float float2_t __attribute__((ext_vector_type(2)));
 
@@ -661,25 +661,50 @@ __device__ __forceinline__ void cmac(float2_t v, float2_t x, bool valid, float& 
   im = valid ? im + pi : im;
 }
 
-// Entries first, first + step, ... < end of the LDS stage, kCsrBatch at a time.
+// One batch: the gathers of entries e0, e0 + step, ... (B of them, those at or
+// past end masked) are all issued before the first is used; the coefficients
+// are read from LDS only when the products are formed, so a batch holds just
+// its gathered rows in registers.
+template <bool CONJ, int B>
+__device__ __forceinline__ void csr_batch(const int32_t* s_col, const float2_t* s_val, uint32_t e0, uint32_t end,
+                                          uint32_t step, rsrc_t in, uint32_t lane_off, uint32_t row_bytes,
+                                          float& re, float& im) {
+  float2_t x[B];
+#pragma unroll
+  for (int i = 0; i < B; ++i) {
+    const uint32_t e = e0 + i * step;
+    const uint32_t col = static_cast<uint32_t>(s_col[e < end ? e : e0]);
+    x[i] = __builtin_bit_cast(float2_t, __builtin_amdgcn_raw_buffer_load_b64(in, col * row_bytes + lane_off, 0, 0));
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int i = 0; i < B; ++i) {
+    const uint32_t e = e0 + i * step;
+    cmac<CONJ>(s_val[e < end ? e : e0], x[i], e < end, re, im);
+  }
+}
+
+// Entries first, first + step, ... < end of the LDS stage, in entry order:
+// batches of 24 while more than 16 remain, then of 16 or 8 (a 36-entry row:
+// 24 + 16, two memory round trips; 24 keeps the kernel at 5 waves per SIMD,
+// where 32 needed 112 registers and 4 waves, measured no faster).
 template <bool CONJ>
 __device__ __forceinline__ void csr_row_sum(const int32_t* s_col, const float2_t* s_val, uint32_t first, uint32_t end,
                                             uint32_t step, rsrc_t in, uint32_t lane_off, uint32_t row_bytes,
                                             float& re, float& im) {
-  for (uint32_t e0 = first; e0 < end; e0 += kCsrBatch * step) {
-    float2_t v[kCsrBatch], x[kCsrBatch];
-    bool ok[kCsrBatch];
-#pragma unroll
-    for (int i = 0; i < kCsrBatch; ++i) {
-      const uint32_t e = e0 + i * step;
-      ok[i] = e < end;
-      const uint32_t ec = ok[i] ? e : first;
-      v[i] = s_val[ec];
-      const uint32_t col = static_cast<uint32_t>(s_col[ec]);
-      x[i] = __builtin_bit_cast(float2_t, __builtin_amdgcn_raw_buffer_load_b64(in, col * row_bytes + lane_off, 0, 0));
+  uint32_t e0 = first;
+  while (e0 < end) {
+    const uint32_t left = (end - e0 + step - 1) / step;
+    if (left > 16) {
+      csr_batch<CONJ, 24>(s_col, s_val, e0, end, step, in, lane_off, row_bytes, re, im);
+      e0 += 24 * step;
+    } else if (left > 8) {
+      csr_batch<CONJ, 16>(s_col, s_val, e0, end, step, in, lane_off, row_bytes, re, im);
+      e0 += 16 * step;
+    } else {
+      csr_batch<CONJ, 8>(s_col, s_val, e0, end, step, in, lane_off, row_bytes, re, im);
+      e0 += 8 * step;
     }
-#pragma unroll
-    for (int i = 0; i < kCsrBatch; ++i) cmac<CONJ>(v[i], x[i], ok[i], re, im);
   }
 }
 
@@ -1895,6 +1920,42 @@ int tsn_direction_upload_csr(const tsn_kb_matrix* kb, int view, tsn_direction** 
     h.output_size = grid;
     h.n_cols = m;
     h.active_rows = static_cast<uint32_t>(rows.size());
+  }
+  // Rows in Morton order of their k-space position (samples: the rounded grid
+  // coordinate; grid points: their own), so the contiguous range of work
+  // items each XCD runs covers a compact k-space region and its gathers stay
+  // in that XCD's L2.  Entries keep their order, so the sums do not change.
+  {
+    const uint32_t n = static_cast<uint32_t>(rows.size());
+    std::vector<uint32_t> key(n);
+    for (uint32_t t = 0; t < n; ++t) {
+      if (view == 0) {
+        const double c0 = std::nearbyint(kb->coord0[t]), c1 = std::nearbyint(kb->coord1[t]);
+        key[t] = morton_key(static_cast<uint32_t>(std::max(0.0, c0)), static_cast<uint32_t>(std::max(0.0, c1)));
+      } else {
+        const uint32_t g = static_cast<uint32_t>(rows[t]);
+        key[t] = morton_key(g % kb->grid1, g / kb->grid1);
+      }
+    }
+    std::vector<uint32_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0u);
+    std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+    std::vector<uint32_t> nptr(1, 0);
+    std::vector<int32_t> nrows(n), ncols(cols.size());
+    std::vector<float2> nvals(vals.size());
+    nptr.reserve(n + 1);
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t t = perm[i];
+      nrows[i] = rows[t];
+      const uint32_t b = ptr[t], e = ptr[t + 1], o = nptr.back();
+      std::copy(cols.begin() + b, cols.begin() + e, ncols.begin() + o);
+      std::copy(vals.begin() + b, vals.begin() + e, nvals.begin() + o);
+      nptr.push_back(o + (e - b));
+    }
+    ptr.swap(nptr);
+    rows.swap(nrows);
+    cols.swap(ncols);
+    vals.swap(nvals);
   }
   uint32_t slots = 0, counters = 0;
   const std::vector<CsrItem> items = csr_items(ptr, &slots, &counters);

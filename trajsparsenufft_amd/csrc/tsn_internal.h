@@ -69,6 +69,19 @@ __host__ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nb) {
   return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
+// Morton (Z-order) key of a 2-D point with 16-bit coordinates.
+inline uint32_t morton_key(uint32_t x, uint32_t y) {
+  auto spread = [](uint32_t v) {
+    v &= 0xFFFFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+  };
+  return spread(x) | (spread(y) << 1);
+}
+
 // bfloat16 bits of v, round to nearest even: the top half of the float32 bits
 // after adding 0x7FFF plus the lowest kept bit; NaN stays a quiet NaN.
 __host__ __device__ inline uint16_t bf16_rne_bits(float v) {
