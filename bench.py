@@ -358,7 +358,7 @@ def interp_roofline(torch, packs, steps, warmup, log):
     res = {}
     for name, ms, n_in, n_out, conj, om in (("forward", fwd_ms, g, m, "false", 3), ("adjoint", adj_ms, m, g, "true", 1)):
         b = nnz * 8 + 2 * n_in * 2 * COILS * 2 + n_out * COILS * 8
-        kernel = f"interp_tiles_kernel<0, 1, true, {om}, {conj}, false>"
+        kernel = f"interp_tiles_kernel<0, 1, true, {om}, {conj}, false, 1>"
         res[name] = {"kernel": kernel, "us": ms * 1e3, "bytes": b, "GBs": b / (ms * 1e-3) / 1e9,
                      "traffic": measured_traffic(kernel)}
         log(f"[bench] interp {name} {ms * 1e3:.1f} us ({res[name]['GBs']:.0f} GB/s algorithmic, "
