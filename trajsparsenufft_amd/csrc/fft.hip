@@ -205,27 +205,31 @@ __global__ __launch_bounds__(512) void fwd_axis1_kernel(FftArgs args) {
     emit_exchange<R, false>(out, lds, lane, w);
     __syncthreads();
     if constexpr (OUT == 0 || OUT == 2) {
-      // four threads per panel row, one uint4 each (words 4q .. 4q + 3: Re for
-      // q < 2, else Im, of coils 4 (q & 1) .. + 3; hi | res << 16), so a wave's
-      // store covers 16 whole 64-byte rows
+      // two threads per panel row, one per coil half h (coils 4h .. 4h + 3):
+      // the Re words 4h .. 4h + 3 and the Im words 8 + 4h .. 8 + 4h + 3 (hi |
+      // res << 16) as two uint4 stores, each exchange element read once.
+      // Thread e takes row (e >> 5) * 16 + (e & 15) and half (e >> 4) & 1, so
+      // the 16 lanes of a group read one slot of 16 rows: distinct banks under
+      // the exchange swizzle.
       uint4* base = reinterpret_cast<uint4*>(a.panel_il + (chunk * grid_rows + static_cast<uint64_t>(k0) * K) * 16);
-      for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(K) * 4; e += blockDim.x) {
-        const uint32_t k1 = e >> 2, q = e & 3;
-        uint32_t w4[4];
+#pragma unroll
+      for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(K) * 2; e += 512u) {  // 512-thread blocks
+        const uint32_t h = (e >> 4) & 1u, k1 = ((e >> 5) << 4) | (e & 15u);
+        float2 v[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) v[t] = lds[xslot(k1, 4 * h + t)];
+        uint32_t re[4], im[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          const float2 v = lds[xslot(k1, 4 * (q & 1) + t)];
-          const float x = q < 2 ? v.x : v.y;
           if constexpr (OUT == 2) {
-            w4[t] = split_bf16(x);
+            re[t] = split_bf16(v[t].x);
+            im[t] = split_bf16(v[t].y);
           } else {
-            _Float16 h, r2;
-            split16(x, h, r2);
-            w4[t] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h)) |
-                    (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, r2)) << 16);
+            split16_pair(v[t].x, v[t].y, re[t], im[t]);
           }
         }
-        base[static_cast<size_t>(k1) * 4 + q] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+        base[static_cast<size_t>(k1) * 4 + h] = make_uint4(re[0], re[1], re[2], re[3]);
+        base[static_cast<size_t>(k1) * 4 + 2 + h] = make_uint4(im[0], im[1], im[2], im[3]);
       }
     } else {
       store_rows<K>(a.panel32 + static_cast<uint64_t>(k0) * K * a.cstride + chunk * 8, a.cstride, cw, a.cstride,

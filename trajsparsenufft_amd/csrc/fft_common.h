@@ -183,6 +183,22 @@ __device__ __forceinline__ void split16(float v, _Float16& hi, _Float16& res) {
   res = static_cast<_Float16>(v - static_cast<float>(hi));
 }
 
+// split16 of two values as interleaved words hi | res << 16, with one packed
+// FP16 conversion per pair (the same round-to-nearest-even as split16): wa
+// for a, wb for b.
+__device__ __forceinline__ void split16_pair(float a, float b, uint32_t& wa, uint32_t& wb) {
+  typedef float f32x2 __attribute__((ext_vector_type(2)));
+  typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+  const f32x2 v = {a, b};
+  const f16x2 hi = __builtin_convertvector(v, f16x2);
+  const f32x2 back = __builtin_convertvector(hi, f32x2);
+  const f32x2 rest = {v.x - back.x, v.y - back.y};
+  const f16x2 res = __builtin_convertvector(rest, f16x2);
+  const uint32_t hb = __builtin_bit_cast(uint32_t, hi), rb = __builtin_bit_cast(uint32_t, res);
+  wa = __builtin_amdgcn_perm(rb, hb, 0x05040100u);  // bytes 0-1 of hb, then bytes 0-1 of rb
+  wb = __builtin_amdgcn_perm(rb, hb, 0x07060302u);  // bytes 2-3 of hb, then bytes 2-3 of rb
+}
+
 // 2-point DFT; NZ = number of leading nonzero inputs (1 or 2).
 template <bool INV, int NZ>
 __device__ __forceinline__ void dft2(const float2* a, float2* X) {
@@ -293,11 +309,16 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Per-wave LDS slice (float2 elements): pass-A layout R rows of 65, pass-B/C
-// layout (R*8 tasks) x 9; both padded against bank conflicts.
+// Per-wave LDS slice (float2 elements): pass-A layout R rows of kRowA, pass-B/C
+// layout (R*8 tasks) x 9.  Both are padded against bank conflicts: pass B reads
+// element b0 + 8 b1 of row k1 = lane >> 3 (b0 = lane & 7), and a row stride of
+// 8 (mod 32) float2 puts the eight rows a 16-lane group touches in distinct
+// banks (stride 65, 1 mod 32, made two lanes of each group share a bank);
+// pass C reads task * 9 + b0, consecutive tasks 9 apart.
+constexpr int kRowA = 72;
 template <int R>
 __host__ __device__ constexpr int wave_lds_elems() {
-  return (R * 72 > R * 65 ? R * 72 : R * 65);
+  return R * kRowA > R * 8 * 9 ? R * kRowA : R * 8 * 9;
 }
 
 // Twiddles of one lane for wave_fft, loaded before the data so their latency
@@ -354,7 +375,7 @@ __device__ __forceinline__ void wave_fft(float2 (&x)[R], float2* buf, const floa
 #pragma unroll
   for (int k1 = 1; k1 < R; ++k1) y[k1] = twiddle<INV>(y[k1], R <= 8 ? twp.a[k1] : tw[lane * k1]);
 #pragma unroll
-  for (int k1 = 0; k1 < R; ++k1) buf[k1 * 65 + lane] = y[k1];
+  for (int k1 = 0; k1 < R; ++k1) buf[k1 * kRowA + lane] = y[k1];
   wave_lds_sync();
   // pass B: task (k1, b0): radix-8 over b1, twiddle W_64^{b0*k2a}
   float2 u[TPL][8];
@@ -364,7 +385,7 @@ __device__ __forceinline__ void wave_fft(float2 (&x)[R], float2* buf, const floa
     if (task < R * 8) {
       const int k1 = task >> 3, b0 = task & 7;
 #pragma unroll
-      for (int b1 = 0; b1 < 8; ++b1) u[s][b1] = buf[k1 * 65 + b0 + 8 * b1];
+      for (int b1 = 0; b1 < 8; ++b1) u[s][b1] = buf[k1 * kRowA + b0 + 8 * b1];
     }
   }
   wave_lds_sync();
@@ -457,14 +478,15 @@ template <int ROWS>
 __device__ __forceinline__ void store_rows(float2* __restrict__ dst, uint64_t row_stride, uint32_t cw,
                                            uint32_t cstride, const float2* lds) {
   if (cw == 8 && (cstride & 1u) == 0) {
-    for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(ROWS) * 4; e += blockDim.x) {
+#pragma unroll
+    for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(ROWS) * 4; e += 512u) {
       const uint32_t r = e >> 2, q = e & 3;
       const float2 u = lds[xslot(static_cast<int>(r), static_cast<int>(2 * q))];
       const float2 v = lds[xslot(static_cast<int>(r), static_cast<int>(2 * q + 1))];
       reinterpret_cast<float4*>(dst + r * row_stride)[q] = make_float4(u.x, u.y, v.x, v.y);
     }
   } else {
-    for (uint32_t r = threadIdx.x; r < static_cast<uint32_t>(ROWS); r += blockDim.x)
+    for (uint32_t r = threadIdx.x; r < static_cast<uint32_t>(ROWS); r += 512u)
       for (uint32_t i = 0; i < cw; ++i) dst[r * row_stride + i] = lds[xslot(static_cast<int>(r), static_cast<int>(i))];
   }
 }
