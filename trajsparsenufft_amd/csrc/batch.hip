@@ -165,10 +165,11 @@ FftArgs batch_fft_args(const tsn_batch* bt) {
   a.panel_il = bt->pan_il;
   a.grid = bt->grid_buf;
   a.t_inv = bt->t_inv;
+  a.grid_blocked = 1;  // each problem's rows compact (<= 8 coils) or in 8-coil blocks
   a.image_stride = bt->pixels;
   a.smaps_stride = static_cast<uint64_t>(bt->pixels) * bt->cmax;
-  a.t_stride = static_cast<uint64_t>(bt->cfg.grid0) * bt->cfg.image1 * bt->cmax;
-  a.grid_stride = static_cast<uint64_t>(bt->grid) * bt->cmax;
+  a.t_stride = static_cast<uint64_t>(bt->cfg.grid0) * bt->cfg.image1 * bt->bmax * kCoilsPerBlock;
+  a.grid_stride = static_cast<uint64_t>(bt->grid) * bt->bmax * kCoilsPerBlock;
   a.panel_stride = panel_stride(bt);
   a.partial_stride = 3 * static_cast<uint64_t>(bt->cfg.image1);
   a.hist_stride = bt->hist_cap;
@@ -450,9 +451,12 @@ int tsn_batch_create(const tsn_plan_config* cfg, const tsn_batch_problem* proble
   A(&bt->tw1, c.grid1);
   A(&bt->smaps_t, S * px * bt->cmax);
   A(&bt->apod_t, px);
-  A(&bt->t_fwd, S * c.grid0 * c.image1 * bt->cmax);
-  A(&bt->t_inv, S * c.grid0 * c.image1 * bt->cmax);
-  A(&bt->grid_buf, S * grid * bt->cmax);
+  // T arrays and grids: per problem, compact rows of its coils (<= 8) or 64-byte
+  // rows in 8-coil blocks, at a common problem stride
+  const size_t slots = static_cast<size_t>(bt->bmax) * kCoilsPerBlock;
+  A(&bt->t_fwd, S * c.grid0 * c.image1 * slots);
+  A(&bt->t_inv, S * c.grid0 * c.image1 * slots);
+  A(&bt->grid_buf, S * grid * slots);
   A(&bt->pan_il, S * panel_stride(bt));
   A(&bt->smp_il, S * sample_stride(bt));
   for (float2** v : {&bt->x, &bt->r, &bt->r2, &bt->p, &bt->p2, &bt->ap}) A(v, S * px);
@@ -474,7 +478,7 @@ int tsn_batch_create(const tsn_plan_config* cfg, const tsn_batch_problem* proble
       rc = fail(TSN_ERR_RUNTIME, "batch setup upload failed");
   }
   if (rc == TSN_OK) {
-    rc = zero_async(bt->grid_buf, S * grid * bt->cmax * sizeof(float2), 0);
+    rc = zero_async(bt->grid_buf, S * grid * slots * sizeof(float2), 0);
     if (rc == TSN_OK) rc = zero_async(bt->smaps_t, S * px * bt->cmax * sizeof(float2), 0);
   }
   for (uint32_t z = 0; rc == TSN_OK && z < count; ++z) {
@@ -497,9 +501,9 @@ int tsn_batch_create(const tsn_plan_config* cfg, const tsn_batch_problem* proble
       const uint32_t blocks = div_up(bt->coils[z], kCoilsPerBlock);
       uint32_t* pan = bt->pan_il + z * panel_stride(bt);
       uint32_t* smp = bt->smp_il + z * sample_stride(bt);
-      float* grid_out = reinterpret_cast<float*>(bt->grid_buf + z * grid * bt->cmax);
-      fw[blocks - 1].push_back({bt->fwd[z], pan, nullptr, smp, bt->cmax, 0, blocks});
-      ad[blocks - 1].push_back({bt->adj[z], smp, grid_out, nullptr, bt->cmax, 0, blocks});
+      float* grid_out = reinterpret_cast<float*>(bt->grid_buf + z * grid * slots);
+      fw[blocks - 1].push_back({bt->fwd[z], pan, nullptr, smp, bt->coils[z], 0, blocks});
+      ad[blocks - 1].push_back({bt->adj[z], smp, grid_out, nullptr, bt->coils[z], 0, blocks});
     }
     for (int k = 0; k < 2; ++k) {
       bt->parts_fwd[k] = fw[k];

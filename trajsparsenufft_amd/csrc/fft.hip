@@ -152,8 +152,9 @@ __global__ __launch_bounds__(512) void fwd_axis0_kernel(FftArgs args) {
   __syncthreads();
   emit_exchange<R, false>(out, lds, lane, w);
   __syncthreads();
-  store_rows<K>(a.t_fwd + static_cast<size_t>(n1) * a.cstride + chunk * 8,
-                static_cast<uint64_t>(a.n1) * a.cstride, min(8u, a.coils - chunk * 8), a.cstride, lds);
+  const ChunkRows tl = chunk_rows(a.coils, static_cast<uint64_t>(a.k0) * a.n1);
+  store_rows<K>(a.t_fwd + chunk * tl.cs + static_cast<size_t>(n1) * tl.rs, static_cast<uint64_t>(a.n1) * tl.rs, tl.rs,
+                lds);
 }
 
 // The second-axis passes can take rows_per_block consecutive grid rows per
@@ -182,15 +183,16 @@ __global__ __launch_bounds__(512) void fwd_axis1_kernel(FftArgs args) {
   const uint32_t cw = min(8u, a.coils - chunk * 8);
   const uint64_t grid_rows = static_cast<uint64_t>(a.k0) * K;
   // grid row k0 of T: N1 = K/2 rows (the plan checks grid = 2 x image)
-  auto t_row = [&](uint32_t k0) { return a.t_fwd + static_cast<size_t>(k0) * a.n1 * a.cstride + chunk * 8; };
+  const ChunkRows tl = chunk_rows(a.coils, static_cast<uint64_t>(a.k0) * a.n1);
+  auto t_row = [&](uint32_t k0) { return a.t_fwd + chunk * tl.cs + static_cast<size_t>(k0) * a.n1 * tl.rs; };
   ExchangeRegs<K / 2> rows;
-  load_exchange<K / 2>(t_row(row0), a.cstride, cw, rows);
+  load_exchange<K / 2>(t_row(row0), tl.rs, cw, rows);
   for (uint32_t i = 0; i < kRowsPerBlock; ++i) {
     const uint32_t k0 = row0 + i;
     if (k0 >= a.k0) break;
     write_exchange<K / 2>(rows, lds);
     __syncthreads();
-    if (i + 1 < kRowsPerBlock && k0 + 1 < a.k0) load_exchange<K / 2>(t_row(k0 + 1), a.cstride, cw, rows);
+    if (i + 1 < kRowsPerBlock && k0 + 1 < a.k0) load_exchange<K / 2>(t_row(k0 + 1), tl.rs, cw, rows);
     float2 x[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -232,8 +234,9 @@ __global__ __launch_bounds__(512) void fwd_axis1_kernel(FftArgs args) {
         base[static_cast<size_t>(k1) * 4 + 2 + h] = make_uint4(im[0], im[1], im[2], im[3]);
       }
     } else {
-      store_rows<K>(a.panel32 + static_cast<uint64_t>(k0) * K * a.cstride + chunk * 8, a.cstride, cw, a.cstride,
-                    lds);
+      const ChunkRows gl = a.grid_blocked ? chunk_rows(a.coils, grid_rows) : ChunkRows{a.cstride, 8};
+      store_rows<K>(a.panel32 + chunk * gl.cs + static_cast<uint64_t>(k0) * K * gl.rs, gl.rs,
+                    a.grid_blocked ? gl.rs : cw, lds);
     }
     __syncthreads();  // the exchange area is refilled next
   }
@@ -254,15 +257,17 @@ __global__ __launch_bounds__(512) void inv_axis1_kernel(FftArgs args) {
   constexpr uint32_t kRowsPerBlock = rows_per_block<R>();
   const uint32_t row0 = xcd_remap(blockIdx.x, gridDim.x) * kRowsPerBlock, chunk = blockIdx.y;
   const uint32_t cw = min(8u, a.coils - chunk * 8);
-  auto g_row = [&](uint32_t k0) { return a.grid + static_cast<size_t>(k0) * K * a.cstride + chunk * 8; };
+  const ChunkRows gl = a.grid_blocked ? chunk_rows(a.coils, static_cast<uint64_t>(a.k0) * K) : ChunkRows{a.cstride, 8};
+  const ChunkRows tl = chunk_rows(a.coils, static_cast<uint64_t>(a.n1) * a.k0);
+  auto g_row = [&](uint32_t k0) { return a.grid + chunk * gl.cs + static_cast<size_t>(k0) * K * gl.rs; };
   ExchangeRegs<K> rows;
-  load_exchange<K>(g_row(row0), a.cstride, cw, rows);
+  load_exchange<K>(g_row(row0), gl.rs, cw, rows);
   for (uint32_t i = 0; i < kRowsPerBlock; ++i) {
     const uint32_t k0 = row0 + i;
     if (k0 >= a.k0) break;
     write_exchange<K>(rows, lds);
     __syncthreads();
-    if (i + 1 < kRowsPerBlock && k0 + 1 < a.k0) load_exchange<K>(g_row(k0 + 1), a.cstride, cw, rows);
+    if (i + 1 < kRowsPerBlock && k0 + 1 < a.k0) load_exchange<K>(g_row(k0 + 1), gl.rs, cw, rows);
     float2 x[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) x[j] = lds[xslot(lane + 64 * j, w)];
@@ -273,8 +278,8 @@ __global__ __launch_bounds__(512) void inv_axis1_kernel(FftArgs args) {
     __syncthreads();
     emit_exchange<R, true>(out, lds, lane, w);
     __syncthreads();
-    store_rows<K / 2>(a.t_inv + static_cast<size_t>(k0) * a.cstride + chunk * 8,
-                      static_cast<uint64_t>(a.k0) * a.cstride, cw, a.cstride, lds);
+    store_rows<K / 2>(a.t_inv + chunk * tl.cs + static_cast<size_t>(k0) * tl.rs, static_cast<uint64_t>(a.k0) * tl.rs,
+                      tl.rs, lds);
     __syncthreads();  // the exchange area is refilled next
   }
 }
@@ -317,7 +322,8 @@ __global__ __launch_bounds__(512) void inv_axis0_kernel(FftArgs args) {
       for (int k2b = 0; k2b < 4; ++k2b)
         smv[s][k2b] = task < R * 8 ? sm[k1 + R * (k2a + 8 * k2b)] : make_float2(0.f, 0.f);
     }
-    fill_exchange<K>(a.t_inv + static_cast<size_t>(n1) * K * a.cstride + chunk * 8, a.cstride, cw, lds);
+    const ChunkRows tl = chunk_rows(a.coils, static_cast<uint64_t>(a.n1) * a.k0);
+    fill_exchange<K>(a.t_inv + chunk * tl.cs + static_cast<size_t>(n1) * K * tl.rs, tl.rs, cw, lds);
     __syncthreads();
     float2 x[R];
 #pragma unroll

@@ -18,11 +18,16 @@ struct FftArgs {
   const float2* image_t;   // [n1][n0]
   const float2* smaps_t;   // [n1][coil][n0]
   const float* apod_t;     // [n1][n0]
-  float2* t_fwd;           // [K0][n1][coil]
+  // Channel layouts (float2; see chunk_rows): t_fwd, t_inv and, with
+  // grid_blocked, grid and panel32 hold each problem's rows compact (its
+  // coils, up to 8) or in 8-coil chunk blocks of 64-byte rows; without
+  // grid_blocked, grid and panel32 rows are cstride wide.
+  float2* t_fwd;           // rows [K0][n1]
   uint32_t* panel_il;      // [block][K0*K1][16] hi | res << 16
-  float2* panel32;         // [K0*K1][coil]
-  const float2* grid;      // [K0*K1][coil]
-  float2* t_inv;           // [n1][K0][coil]
+  float2* panel32;         // rows [K0*K1]
+  const float2* grid;      // rows [K0*K1]
+  float2* t_inv;           // rows [n1][K0]
+  uint32_t grid_blocked;
   float2* image_out;       // [n1][n0]
   const float2* reg_in;    // [n1][n0] or null
   float lambda;
@@ -116,6 +121,21 @@ __device__ __forceinline__ void sum_partials3(const double* __restrict__ p, uint
     c += z;
   }
   block_sum3<WAVES>(a, b, c, red);
+}
+
+// Where coil chunk `chunk` of row r starts (float2): chunk * cs + r * rs.  A
+// problem of up to 8 coils keeps rows of exactly its coils (one chunk); above
+// 8, every chunk is a block of rows of 8 (64 bytes, whole L2 sectors: a
+// chunk never reads or writes part of a line that another chunk's block
+// touches).  The legacy layout (grid and FP32 panel of plans whose grid the
+// FP32 CSR kernel reads or writes) has rows of cstride with chunk c at
+// column 8c.
+struct ChunkRows {
+  uint32_t rs;  // between rows
+  uint64_t cs;  // between chunks
+};
+__device__ __forceinline__ ChunkRows chunk_rows(uint32_t coils, uint64_t rows) {
+  return coils <= 8 ? ChunkRows{coils, 0} : ChunkRows{8u, rows * 8};
 }
 
 // The arguments of problem z of a batched launch.
@@ -469,25 +489,28 @@ __device__ __forceinline__ void fill_exchange(const float2* __restrict__ src, ui
   write_exchange<ROWS>(r, lds);
 }
 
-// Store ROWS rows of the exchange area (8 coils each, coils >= cw skipped) to
-// dst + r * row_stride (float2 units).  Whole 8-coil rows at a 16-byte
-// aligned stride go out as four float4 per row from four neighbouring
-// threads, so a wave's store covers 16 whole 64-byte rows; otherwise one
-// thread writes a row coil by coil.  512-thread blocks.
+// Store the first `slots` coils of ROWS rows of the exchange area to dst + r
+// * row_stride (float2 units; slots <= 8; exchange slots past a chunk's
+// coils hold zeros).  An even slot count at an even stride (16-byte aligned
+// rows) goes out as float4 coil pairs, four neighbouring threads per row, so
+// a wave's store covers 16 whole rows; otherwise one thread writes a row coil
+// by coil.  512-thread blocks.
 template <int ROWS>
-__device__ __forceinline__ void store_rows(float2* __restrict__ dst, uint64_t row_stride, uint32_t cw,
-                                           uint32_t cstride, const float2* lds) {
-  if (cw == 8 && (cstride & 1u) == 0) {
+__device__ __forceinline__ void store_rows(float2* __restrict__ dst, uint64_t row_stride, uint32_t slots,
+                                           const float2* lds) {
+  if ((slots & 1u) == 0 && (row_stride & 1u) == 0) {
 #pragma unroll
     for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(ROWS) * 4; e += 512u) {
       const uint32_t r = e >> 2, q = e & 3;
+      if (2 * q >= slots) continue;
       const float2 u = lds[xslot(static_cast<int>(r), static_cast<int>(2 * q))];
       const float2 v = lds[xslot(static_cast<int>(r), static_cast<int>(2 * q + 1))];
       reinterpret_cast<float4*>(dst + r * row_stride)[q] = make_float4(u.x, u.y, v.x, v.y);
     }
   } else {
     for (uint32_t r = threadIdx.x; r < static_cast<uint32_t>(ROWS); r += 512u)
-      for (uint32_t i = 0; i < cw; ++i) dst[r * row_stride + i] = lds[xslot(static_cast<int>(r), static_cast<int>(i))];
+      for (uint32_t i = 0; i < slots; ++i)
+        dst[r * row_stride + i] = lds[xslot(static_cast<int>(r), static_cast<int>(i))];
   }
 }
 

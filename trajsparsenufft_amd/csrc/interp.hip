@@ -31,7 +31,11 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 
 namespace tsn {
 
-enum OutMode { kOutCoilMajor = 0, kOutRowMajor = 1, kOutSplit16 = 2, kOutSplit16IL = 3, kOutSplitBF16IL = 4 };
+// kOutBlockRows (host side only): row-major float2 in blocks of 8 coils, block
+// b of rows at b * n_out * 8 (64-byte rows), launched as kOutRowMajor with
+// the block stride set.
+enum OutMode { kOutCoilMajor = 0, kOutRowMajor = 1, kOutSplit16 = 2, kOutSplit16IL = 3, kOutSplitBF16IL = 4,
+               kOutBlockRows = 5 };
 // Tile kinds: FP16 tiles x FP16 hi + residual on smfmac, the same on dense
 // MFMA, FP16 hi only, and BF16 tiles x BF16 hi + residual on smfmac.
 enum TileKind { kSparseSplit = 0, kDenseSplit = 1, kSparseHi = 2, kSparseSplitBF16 = 3 };
@@ -63,6 +67,10 @@ struct InterpArgs {
   _Float16* out_hi;       // split output panel blocks
   _Float16* out_res;
   uint32_t* out_il;       // interleaved split output panel blocks
+  // row-major float2 output: (row, coil) at float2 (coil >> 3) * out_bs +
+  // row * out_rs + (coil & 7) ([row][coils]: out_rs = coils, out_bs = 8)
+  uint32_t out_rs;
+  uint64_t out_bs;
   uint32_t coil_base;     // first coil of block 0 in the output (batched launches)
   // Batched launches (one coil block of one problem per entry; NB = 1,
   // interleaved panels): items[i] belongs to problems[item_problem[i]].
@@ -90,6 +98,8 @@ struct InterpProblem {
   const uint32_t* panel_il;  // this block's input panel [n_cols][16]
   float* out_f32;       // the problem's float2 output
   uint32_t* out_il;     // this block's split output panel [n_out][16]
+  uint32_t out_rs;      // row-major output strides (as InterpArgs)
+  uint64_t out_bs;
 };
 
 // Global-space view of a plain pointer.  The kernel's loads and stores after
@@ -119,6 +129,8 @@ __device__ __forceinline__ InterpArgs problem_args(const InterpArgs& a, uint32_t
   v.panel_stride = static_cast<uint64_t>(q.n_cols) * kN;  // block b's input panel follows at b strides
   v.out_f32 = q.out_f32;
   v.out_il = q.out_il;
+  v.out_rs = q.out_rs;
+  v.out_bs = q.out_bs;
   return v;
 }
 
@@ -190,7 +202,7 @@ __device__ __forceinline__ void interp_epilogue(const InterpArgs& a, const int4&
       if (coil >= a.coils) continue;
       size_t o;
       if constexpr (OM == kOutCoilMajor) o = (static_cast<size_t>(coil) * a.n_out + row) * 2 + part;
-      else o = (static_cast<size_t>(row) * a.coils + coil) * 2 + part;
+      else o = ((coil >> 3) * a.out_bs + static_cast<size_t>(row) * a.out_rs + (coil & 7)) * 2 + part;
       if constexpr (ACC) v += glb(a.out_f32)[o];
       glb(a.out_f32)[o] = v;
     }
@@ -972,6 +984,8 @@ static int launch_interp_impl(const tsn_direction* d, int kind, const void* hi, 
     return fail(TSN_ERR_ARGUMENT, "coils must be in 1..16 (got %u)", coils);
   if (!hi || (!interleaved && kind != kSparseHi && !res))
     return fail(TSN_ERR_ARGUMENT, "missing input panel");
+  const bool block_rows = out_mode == kOutBlockRows;
+  if (block_rows) out_mode = kOutRowMajor;
   const bool split_out = out_mode == kOutSplit16 || out_mode == kOutSplit16IL;
   if (split_out ? (interleaved ? !out_il : (!out_hi || !out_res)) : !out_f32)
     return fail(TSN_ERR_ARGUMENT, "missing output");
@@ -1011,6 +1025,8 @@ static int launch_interp_impl(const tsn_direction* d, int kind, const void* hi, 
   a.out_hi = out_hi;
   a.out_res = out_res;
   a.out_il = out_il;
+  a.out_rs = block_rows ? kCoilsPerBlock : coils;
+  a.out_bs = block_rows ? static_cast<uint64_t>(a.n_out) * kCoilsPerBlock : kCoilsPerBlock;
   if (dc) {
     a.dc_meas = dc->y;
     a.dc_density = dc->density;
@@ -1545,6 +1561,10 @@ int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* o
     q.panel_il = p.panel_il;
     q.out_f32 = p.out_f32;
     q.out_il = p.out_il;
+    // row-major output: compact rows of the problem's coils (one block), or
+    // 64-byte rows in blocks of 8 coils (two blocks)
+    q.out_rs = blocks == 2 ? kCoilsPerBlock : p.coils;
+    q.out_bs = blocks == 2 ? static_cast<uint64_t>(q.n_out) * kCoilsPerBlock : kCoilsPerBlock;
     probs.push_back(q);
   }
   // Parts are grouped by direction, in order of first appearance.  Each

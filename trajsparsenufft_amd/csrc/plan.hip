@@ -94,7 +94,17 @@ FftArgs fft_args(const tsn_plan* pl) {
   a.panel32 = pl->pan32;
   a.grid = pl->grid;
   a.t_inv = pl->t_inv;
+  // the packed variants' adjoint writes the grid in the FFT's chunk layout;
+  // the FP32 CSR kernels gather and write [row][coils] rows
+  a.grid_blocked = pl->cfg.precision == TSN_PREC_FP32 ? 0u : 1u;
   return a;
+}
+
+// float2 per row of the T arrays (and of the packed variants' grid): a
+// problem's own coils up to 8, else 8 per coil block
+uint32_t chunk_slots(uint32_t coils) {
+  constexpr uint32_t kB = kCoilsPerBlock;
+  return coils <= kB ? coils : div_up(coils, kB) * kB;
 }
 
 bool is_fp32(const tsn_plan* pl) { return pl->cfg.precision == TSN_PREC_FP32; }
@@ -145,8 +155,10 @@ int op_adjoint_grid(tsn_plan* pl, hipStream_t s) {
   if (is_fp32(pl))
     return launch_interp_csr(pl->adj, pl->smp32, pl->cfg.coils, 1, true, false, 1.0f, pl->grid, s,
                              &pl->adj_chunks);
-  return launch_interp(pl->adj, interp_kind(pl), pl->smp_il, nullptr, pl->cfg.coils, 1, true, false, 1.0f,
-                       reinterpret_cast<float*>(pl->grid), nullptr, nullptr, s, true, nullptr, &pl->adj_chunks);
+  // row-major (mode 1) up to 8 coils, 8-coil blocks of rows (mode 5) above
+  return launch_interp(pl->adj, interp_kind(pl), pl->smp_il, nullptr, pl->cfg.coils,
+                       pl->cfg.coils > static_cast<uint32_t>(kCoilsPerBlock) ? 5 : 1, true, false, 1.0f, reinterpret_cast<float*>(pl->grid),
+                       nullptr, nullptr, s, true, nullptr, &pl->adj_chunks);
 }
 
 // grid -> image (column-major) with combine, scale, regulariser, dot partial
@@ -328,9 +340,10 @@ int tsn_plan_create(const tsn_plan_config* cfg, const tsn_direction* fwd, const 
   A(&pl->tw1, c.grid1);
   A(&pl->smaps_t, px * c.coils);
   A(&pl->apod_t, px);
-  A(&pl->t_fwd, static_cast<size_t>(c.grid0) * c.image1 * c.coils);
-  A(&pl->t_inv, static_cast<size_t>(c.image1) * c.grid0 * c.coils);
-  A(&pl->grid, static_cast<size_t>(grid) * c.coils);
+  const size_t grid_slots = fp32 ? c.coils : chunk_slots(c.coils);
+  A(&pl->t_fwd, static_cast<size_t>(c.grid0) * c.image1 * chunk_slots(c.coils));
+  A(&pl->t_inv, static_cast<size_t>(c.image1) * c.grid0 * chunk_slots(c.coils));
+  A(&pl->grid, static_cast<size_t>(grid) * grid_slots);
   if (fp32) {
     A(&pl->pan32, static_cast<size_t>(grid) * c.coils);
     A(&pl->smp32, static_cast<size_t>(pl->m) * c.coils);
@@ -363,7 +376,7 @@ int tsn_plan_create(const tsn_plan_config* cfg, const tsn_direction* fwd, const 
     hipLaunchKernelGGL(apod_transpose_kernel, dim3(div_up(px, 256)), dim3(256), 0, 0,
                        static_cast<const float*>(apodization), c.image0, c.image1, pl->apod_t);
     // the adjoint grid's inactive rows are never written: zero it once here
-    rc = zero_async(pl->grid, static_cast<size_t>(grid) * c.coils * sizeof(float2), 0);
+    rc = zero_async(pl->grid, static_cast<size_t>(grid) * grid_slots * sizeof(float2), 0);
     // arrival counters start at zero (each launch's last arrivals reset them)
     for (const ChunkScratch* k : {&pl->fwd_chunks, &pl->adj_chunks})
       if (rc == TSN_OK && k->counters)

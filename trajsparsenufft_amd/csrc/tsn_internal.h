@@ -185,7 +185,8 @@ std::vector<ChunkItem> long_chunks(const std::vector<uint32_t>& group_offsets, u
 struct InterpBatchPart {
   const tsn_direction* dir;
   const uint32_t* panel_il;  // the first block's interleaved input panel [n_cols][16]
-  float* out_f32;            // the problem's float2 output (row- or coil-major)
+  float* out_f32;            // the problem's float2 output (row-major: compact rows of its
+                             // coils up to 8 coils, 8-coil blocks of rows above 8)
   uint32_t* out_il;          // the first block's interleaved split output panel [n_out][16]
   uint32_t coils;            // coils of the problem
   uint32_t coil_base;        // first coil of the first block
