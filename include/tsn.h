@@ -159,8 +159,9 @@ int tsn_direction_set_coil_waves(tsn_direction* dir, uint32_t waves);
 uint32_t tsn_direction_coil_waves(const tsn_direction* dir);
 /* Autotuner: time the interpolation of this direction for `coils` coils over
  * the candidate unit sizes {2,3,4,5,6,8,10,12,14,16}, and for 9..16 coils
- * over both coil-wave settings (scratch panels allocated and freed here, the
- * stream synchronized), and keep the fastest schedule.  Results do not depend
+ * over both coil-wave settings, writing 64-byte rows per coil block as plans
+ * do (scratch panels allocated and freed here, the stream synchronized), and
+ * keep the fastest schedule.  Results do not depend
  * on the schedule (bitwise).  No counterpart in the reference; run once per
  * trajectory and coil count. */
 int tsn_direction_autotune(tsn_direction* dir, uint32_t coils, void* stream, uint32_t* best_unit_tiles,
