@@ -157,13 +157,22 @@ uint32_t tsn_direction_unit_tiles(const tsn_direction* dir);
  * group). */
 int tsn_direction_set_coil_waves(tsn_direction* dir, uint32_t waves);
 uint32_t tsn_direction_coil_waves(const tsn_direction* dir);
+/* Pipeline depth of interleaved-panel interpolation (tiles whose operand
+ * loads are in flight ahead of the multiply): 0 = the kernel's default (2
+ * for one coil block per wave, 1 for two), or 1..3; a depth the launch does
+ * not offer (1 or 3 for one block per wave, 2 for two; plan-style outputs)
+ * falls back to the default.  Results are bitwise the same.  No counterpart
+ * in the reference. */
+int tsn_direction_set_pipe_depth(tsn_direction* dir, uint32_t depth);
+uint32_t tsn_direction_pipe_depth(const tsn_direction* dir);
 /* Autotuner: time the interpolation of this direction for `coils` coils over
- * the candidate unit sizes {2,3,4,5,6,8,10,12,14,16}, and for 9..16 coils
- * over both coil-wave settings, writing 64-byte rows per coil block as plans
- * do (scratch panels allocated and freed here, the stream synchronized), and
- * keep the fastest schedule.  Results do not depend
- * on the schedule (bitwise).  No counterpart in the reference; run once per
- * trajectory and coil count. */
+ * the candidate unit sizes {2,3,4,5,6,8,10,12,14,16} x pipeline depths
+ * (default and alternatives), and for 9..16 coils over both coil-wave
+ * settings, writing 64-byte rows per coil block as plans do (scratch panels
+ * allocated and freed here, the stream synchronized; two passes, the faster
+ * of each candidate counts), and keep the fastest schedule.  Results do not
+ * depend on the schedule (bitwise).  No counterpart in the reference; run
+ * once per trajectory and coil count. */
 int tsn_direction_autotune(tsn_direction* dir, uint32_t coils, void* stream, uint32_t* best_unit_tiles,
                            float* best_us);
 /* Device bytes the direction occupies (pack + index arrays, plus the
@@ -336,13 +345,15 @@ size_t tsn_batch_bytes(const tsn_batch* batch);
  * read during the call (device tables are staged after draining the stream). */
 /* Batch autotuner: time the batch's interpolation steps (every problem's
  * forward and adjoint launches) over the member group of frames sharing a
- * pack (how many of them follow each work entry: 1, 2, 4, 8, 16 or all) and,
- * when problems of 9..16 coils are present, over 1 or 2 waves per entry; keep
- * the fastest.  Results are bitwise the same.  Call before capturing a graph
+ * pack (how many of them follow each work entry: 1, 2, 4, 8, 16 or all), the
+ * pipeline depth (0..3, as tsn_direction_set_pipe_depth, one value for all the
+ * batch's launches) and, when problems of 9..16 coils are present, 1 or 2
+ * waves per entry; keep the fastest (two passes, the faster of each candidate
+ * counts).  Results are bitwise the same.  Call before capturing a graph
  * (fails otherwise); synchronizes the stream.  The batch's interpolation
  * buffers hold unspecified values afterwards (every solve rewrites them). */
 int tsn_batch_autotune(tsn_batch* batch, void* stream, uint32_t* best_member_group, uint32_t* best_coil_waves,
-                       float* best_us);
+                       uint32_t* best_depth, float* best_us);
 int tsn_batch_cg(tsn_batch* batch, const void* const* measurements, uint32_t iterations,
                  const float* operator_scales, float lambda, void* const* x_out,
                  void* const* residuals_out, void* status_out, void* stream);

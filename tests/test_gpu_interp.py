@@ -283,6 +283,50 @@ def test_coil_waves_are_bitwise_invariant(coils, layout):
         assert rel_l2(got, ref) <= TOL
 
 
+@pytest.mark.parametrize("coils", [8, 16])
+@pytest.mark.parametrize("role", ["forward", "adjoint"])
+def test_pipe_depths_are_bitwise_invariant(coils, role):
+    """Every pipeline depth (the default and the autotuner's alternatives) of
+    the plan-style launches gives the same bits: forward into an interleaved
+    split panel, adjoint (conjugate) into float2 rows, long groups included;
+    the float2 rows also match the float64 product.  Depths outside 0..3 are
+    refused."""
+
+    from oracle.solver import apply_pack
+    from trajsparsenufft_amd import device
+    from trajsparsenufft_amd._lib import TsnError
+    from trajsparsenufft_amd.device import Direction
+    _, packs = small_case(32, 96)
+    pk = packs.forward if role == "forward" else packs.adjoint
+    arrays, header = pk.arrays(), pk.header
+    d = Direction.upload(pk)
+    rng = np.random.default_rng(coils + 7)
+    v = (rng.standard_normal((coils, header["n_cols"])) + 1j * rng.standard_normal((coils, header["n_cols"])))
+    il = device.split_panel_il(torch.from_numpy(v.astype(np.complex64)).cuda(), coils)
+    rows, blocks = header["output_size"], (coils + 7) // 8
+    conj = role == "adjoint"
+    outs = []
+    for depth in (0, 1, 2, 3, 0):
+        d.pipe_depth = depth
+        assert d.pipe_depth == depth
+        if conj:
+            out = torch.zeros((rows, coils), dtype=torch.complex64, device="cuda")
+            device.interp_il(d, il, out, coils=coils, conjugate=True, scale=0.5, layout="row_major")
+        else:
+            out = torch.zeros((blocks, rows, 16), dtype=torch.int32, device="cuda")
+            device.interp_il(d, il, out, coils=coils, conjugate=False, scale=0.5, layout="split_il")
+        outs.append(out.cpu().numpy())
+    assert all(np.array_equal(outs[0].view(np.uint32), o.view(np.uint32)) for o in outs[1:])
+    if conj:
+        w = il.cpu().numpy().view(np.uint32)
+        hi = (w & 0xFFFF).astype(np.uint16).view(np.float16)
+        res = (w >> 16).astype(np.uint16).view(np.float16)
+        ref = apply_pack(arrays, rows, hi, res, coils, True, 0.5)
+        assert rel_l2(outs[0].T, ref) <= TOL
+    with pytest.raises(TsnError):
+        d.pipe_depth = 4
+
+
 @pytest.mark.parametrize("view", ["G", "G_T"])
 @pytest.mark.parametrize("coils", [8, 16])
 def test_bf16_tiles_and_panels_match_oracle(view, coils):

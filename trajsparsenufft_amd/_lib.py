@@ -74,7 +74,10 @@ SIGNATURES = {
     "tsn_direction_unit_tiles": (C.c_uint32, [P]),
     "tsn_direction_set_coil_waves": (C.c_int, [P, C.c_uint32]),
     "tsn_direction_coil_waves": (C.c_uint32, [P]),
-    "tsn_batch_autotune": (C.c_int, [P, P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
+    "tsn_direction_set_pipe_depth": (C.c_int, [P, C.c_uint32]),
+    "tsn_direction_pipe_depth": (C.c_uint32, [P]),
+    "tsn_batch_autotune": (C.c_int, [P, P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                     C.POINTER(C.c_float)]),
     "tsn_direction_autotune": (C.c_int, [P, C.c_uint32, P, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
     "tsn_direction_free": (None, [P]),
     "tsn_interp": (C.c_int, [P, P, P, P, U32, C.c_int, C.c_int, C.c_float, C.c_int, P]),

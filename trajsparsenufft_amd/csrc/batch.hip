@@ -93,7 +93,7 @@ struct tsn_batch {
   std::vector<const tsn_direction*> fwd, adj;
   InterpBatch ib_fwd[2], ib_adj[2];  // parts of one and of two coil blocks
   std::vector<InterpBatchPart> parts_fwd[2], parts_adj[2];  // (kept for the autotuner's rebuilds)
-  uint32_t member_group = 0, coil_waves = kDefaultCoilWaves;
+  uint32_t member_group = 0, coil_waves = kDefaultCoilWaves, depth = 0;
   std::vector<void*> allocations;
   size_t bytes = 0;
   float2 *tw0 = nullptr, *tw1 = nullptr, *smaps_t = nullptr, *t_fwd = nullptr, *t_inv = nullptr, *grid_buf = nullptr;
@@ -316,8 +316,8 @@ int build_interp_tables(tsn_batch* bt) {
   }
   for (int k = 0; k < 2; ++k) {
     if (bt->parts_fwd[k].empty()) continue;
-    TSN_CHECK(interp_batch_build(bt->parts_fwd[k], &bt->ib_fwd[k], bt->member_group, bt->coil_waves));
-    TSN_CHECK(interp_batch_build(bt->parts_adj[k], &bt->ib_adj[k], bt->member_group, bt->coil_waves));
+    TSN_CHECK(interp_batch_build(bt->parts_fwd[k], &bt->ib_fwd[k], bt->member_group, bt->coil_waves, bt->depth));
+    TSN_CHECK(interp_batch_build(bt->parts_adj[k], &bt->ib_adj[k], bt->member_group, bt->coil_waves, bt->depth));
   }
   return TSN_OK;
 }
@@ -338,7 +338,7 @@ int enqueue_interp_steps(tsn_batch* bt, hipStream_t s) {
 extern "C" {
 
 int tsn_batch_autotune(tsn_batch* bt, void* stream, uint32_t* best_member_group, uint32_t* best_coil_waves,
-                       float* best_us) {
+                       uint32_t* best_depth, float* best_us) {
   if (!bt) return fail(TSN_ERR_ARGUMENT, "null batch");
   if (bt->graph) return fail(TSN_ERR_ARGUMENT, "tune a batch before capturing its graph");
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -349,6 +349,9 @@ int tsn_batch_autotune(tsn_batch* bt, void* stream, uint32_t* best_member_group,
     if (g <= bt->problems && std::find(groups.begin(), groups.end(), g) == groups.end()) groups.push_back(g);
   std::vector<uint32_t> waves = {bt->coil_waves};
   if (!bt->parts_fwd[1].empty()) waves = {1u, 2u};
+  // pipeline depths: the kernels' default and every alternative (one-block
+  // launches take 1 or 3, two-block launches 2; others keep their default)
+  const std::vector<uint32_t> depths = {0u, 1u, 2u, 3u};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = TSN_OK;
   auto hip = [&](hipError_t e, const char* what) {
@@ -357,36 +360,38 @@ int tsn_batch_autotune(tsn_batch* bt, void* stream, uint32_t* best_member_group,
   hip(hipEventCreate(&e0), "event");
   hip(hipEventCreate(&e1), "event");
   // two passes over the candidates, each candidate's faster pass counts
-  std::vector<float> times(waves.size() * groups.size(), 0.0f);
-  uint32_t best_g = 0, best_w = 0;
+  const size_t n_cand = waves.size() * groups.size() * depths.size();
+  std::vector<float> times(n_cand, 0.0f);
+  uint32_t best_g = 0, best_w = 0, best_d = 0;
   float best_ms = 0.0f;
   for (int pass = 0; pass < 2; ++pass)
-  for (size_t wi = 0; wi < waves.size(); ++wi) {
-    for (size_t gi = 0; gi < groups.size(); ++gi) {
-      if (rc != TSN_OK) break;
-      const uint32_t w = waves[wi], g = groups[gi];
-      bt->member_group = g;
-      bt->coil_waves = w;
-      rc = build_interp_tables(bt);
-      for (int i = 0; rc == TSN_OK && i < 2; ++i) rc = enqueue_interp_steps(bt, s);
-      if (rc != TSN_OK) break;
-      hip(hipEventRecord(e0, s), "event");
-      constexpr int kReps = 4;
-      for (int i = 0; rc == TSN_OK && i < kReps; ++i) rc = enqueue_interp_steps(bt, s);
-      hip(hipEventRecord(e1, s), "event");
-      hip(hipEventSynchronize(e1), "event");
-      float ms = 0.0f;
-      hip(hipEventElapsedTime(&ms, e0, e1), "event");
-      ms /= kReps;
-      if (rc != TSN_OK) break;
-      float& t = times[wi * groups.size() + gi];
-      if (pass == 1) ms = std::min(ms, t);
-      t = ms;
-      if (pass == 1 && (best_g == 0 || ms < best_ms)) {
-        best_g = g;
-        best_w = w;
-        best_ms = ms;
-      }
+  for (size_t ci = 0; ci < n_cand; ++ci) {
+    if (rc != TSN_OK) break;
+    const uint32_t w = waves[ci / (groups.size() * depths.size())];
+    const uint32_t g = groups[(ci / depths.size()) % groups.size()];
+    const uint32_t dp = depths[ci % depths.size()];
+    bt->member_group = g;
+    bt->coil_waves = w;
+    bt->depth = dp;
+    rc = build_interp_tables(bt);
+    for (int i = 0; rc == TSN_OK && i < 2; ++i) rc = enqueue_interp_steps(bt, s);
+    if (rc != TSN_OK) break;
+    hip(hipEventRecord(e0, s), "event");
+    constexpr int kReps = 4;
+    for (int i = 0; rc == TSN_OK && i < kReps; ++i) rc = enqueue_interp_steps(bt, s);
+    hip(hipEventRecord(e1, s), "event");
+    hip(hipEventSynchronize(e1), "event");
+    float ms = 0.0f;
+    hip(hipEventElapsedTime(&ms, e0, e1), "event");
+    ms /= kReps;
+    if (rc != TSN_OK) break;
+    if (pass == 1) ms = std::min(ms, times[ci]);
+    times[ci] = ms;
+    if (pass == 1 && (best_g == 0 || ms < best_ms)) {
+      best_g = g;
+      best_w = w;
+      best_d = dp;
+      best_ms = ms;
     }
   }
   (void)hipStreamSynchronize(s);
@@ -395,10 +400,12 @@ int tsn_batch_autotune(tsn_batch* bt, void* stream, uint32_t* best_member_group,
   if (rc != TSN_OK) return rc;
   bt->member_group = best_g;
   bt->coil_waves = best_w;
+  bt->depth = best_d;
   TSN_CHECK(build_interp_tables(bt));
   TSN_HIP(hipStreamSynchronize(s));
   if (best_member_group) *best_member_group = best_g;
   if (best_coil_waves) *best_coil_waves = best_w;
+  if (best_depth) *best_depth = best_d;
   if (best_us) *best_us = best_ms * 1000.0f;
   return TSN_OK;
 }

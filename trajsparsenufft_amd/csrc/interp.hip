@@ -72,6 +72,7 @@ struct InterpArgs {
   uint32_t out_rs;
   uint64_t out_bs;
   uint32_t coil_base;     // first coil of block 0 in the output (batched launches)
+  uint32_t depth;         // tiles in flight ahead of the multiply: 0 = PipeDepth, else 1..3
   // Batched launches (one coil block of one problem per entry; NB = 1,
   // interleaved panels): items[i] belongs to problems[item_problem[i]].
   const struct InterpProblem* problems;
@@ -409,8 +410,9 @@ struct PipeDepth {
 // Stream tiles first .. end - 1 (at most kSplitAboveTiles, column words
 // staged) of one wave through a software pipeline.  Step i reads the column
 // words of tile i + D + 1 from the stage, issues the operand loads of tile
-// i + D, then multiplies tile i (D = PipeDepth); operand loads return in
-// issue order, so the multiply's wait leaves D tiles of loads in flight.
+// i + D, then multiplies tile i (D: PipeDepth, or the autotuner's choice);
+// operand loads return in issue order, so the multiply's wait leaves D tiles
+// of loads in flight.
 // Tile i lives in register slot i % U of both rings (U = max(D + 1, 2), the
 // loop unrolled U times so every slot index is a constant).  Indices past the
 // end are clamped to the last tile (valid, unused loads).  The tiles are
@@ -419,11 +421,10 @@ struct PipeDepth {
 // (loaded one group ahead) and resets the accumulators; with 0 (a chunk of a
 // long group) everything accumulates.  One loop for both keeps the kernel's
 // register count at that of the regular units.
-template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC>
+template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC, int D>
 __device__ __forceinline__ void stream_tiles(const InterpArgs& a, const TileRsrc<NB>& r, int first, int end,
                                              int group, int group_end, uint32_t last_mask, float4_t (&acc_re)[NB],
                                              float4_t (&acc_im)[NB], int lane, const uint32_t* stage) {
-  constexpr int D = PipeDepth<KIND, NB, IL>::value;
   constexpr int U = D + 1 > 2 ? D + 1 : 2;
   // wave-uniform bounds: scalar control flow and scalar tile offsets
   first = __builtin_amdgcn_readfirstlane(first);
@@ -569,8 +570,11 @@ __device__ __forceinline__ void chunk_finish(const InterpArgs& a, const ChunkIte
 // entry, wave w computing coil block w.  Both stream the entry's tile operands
 // (the second read comes from cache: the two waves share a CU), each with the
 // one-block register budget and pipeline, and gather only its own panel.
-template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC, int WB = 1>
-__global__ __launch_bounds__(64 * WB) void interp_tiles_kernel(InterpArgs a) {
+// (the arguments by value: a reference to the kernel's by-value argument
+// would make the compiler copy it to scratch and read it back through flat
+// loads)
+template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC, int WB, int D>
+__device__ __forceinline__ void tiles_entry(InterpArgs a) {
   static_assert(WB == 1 || NB == 1, "one wave per coil block covers one block");
   __shared__ uint32_t stages[WB][kStageWords];
   const int lane = threadIdx.x & 63;
@@ -619,9 +623,20 @@ __global__ __launch_bounds__(64 * WB) void interp_tiles_kernel(InterpArgs a) {
   }
   const TileRsrc<NB> r = make_tile_rsrc<NB, IL>(vw);
   stage_cols(r, u.tile_begin, u.tile_end - u.tile_begin, stage, lane);
-  stream_tiles<KIND, NB, IL, OM, CONJ, ACC>(vw, r, u.tile_begin, u.tile_end, u.group, u.group_end,
-                                            is_chunk ? 0u : 1u, acc_re, acc_im, lane, stage);
+  stream_tiles<KIND, NB, IL, OM, CONJ, ACC, D>(vw, r, u.tile_begin, u.tile_end, u.group, u.group_end,
+                                               is_chunk ? 0u : 1u, acc_re, acc_im, lane, stage);
   if (is_chunk) chunk_finish<OM, CONJ, ACC, NB, WB>(v, c, acc_re, acc_im, lane, wb);
+}
+
+template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC, int WB = 1>
+__global__ __launch_bounds__(64 * WB) void interp_tiles_kernel(InterpArgs a) {
+  tiles_entry<KIND, NB, IL, OM, CONJ, ACC, WB, PipeDepth<KIND, NB, IL>::value>(a);
+}
+
+// The same with another pipeline depth D (the autotuner's alternatives).
+template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC, int WB, int D>
+__global__ __launch_bounds__(64 * WB) void interp_tiles_depth_kernel(InterpArgs a) {
+  tiles_entry<KIND, NB, IL, OM, CONJ, ACC, WB, D>(a);
 }
 
 
@@ -912,11 +927,43 @@ __global__ __launch_bounds__(256) void dc_residual_kernel(const float* __restric
 
 // ------------------------------------------------------------- dispatch
 
+// Pipeline depths the autotuner may pick besides PipeDepth: for the launches
+// plans and batches make with interleaved panels (forward into a split
+// panel, adjoint into float2 rows), one wave per entry: 1 or 3 tiles ahead
+// for one coil block, 2 for two.
+template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC, int WB>
+struct AltDepths {
+  static constexpr bool kPlanLaunch = IL && KIND != kDenseSplit && !ACC && WB == 1 &&
+                                      ((OM == kOutRowMajor && CONJ) ||
+                                       ((OM == kOutSplit16IL || OM == kOutSplitBF16IL) && !CONJ));
+};
+
 template <int KIND, int NB, bool IL, int OM, bool CONJ, bool ACC, int WB>
 static void launch_both(const InterpArgs& a, hipStream_t s) {
   const uint32_t blocks = a.n_chunks + a.n_items;
-  if (blocks)
-    hipLaunchKernelGGL((interp_tiles_kernel<KIND, NB, IL, OM, CONJ, ACC, WB>), dim3(blocks), dim3(64 * WB), 0, s, a);
+  if (!blocks) return;
+  using Alt = AltDepths<KIND, NB, IL, OM, CONJ, ACC, WB>;
+  if constexpr (Alt::kPlanLaunch) {
+    if constexpr (NB == 1) {
+      if (a.depth == 1) {
+        hipLaunchKernelGGL((interp_tiles_depth_kernel<KIND, NB, IL, OM, CONJ, ACC, WB, 1>), dim3(blocks),
+                           dim3(64 * WB), 0, s, a);
+        return;
+      }
+      if (a.depth == 3) {
+        hipLaunchKernelGGL((interp_tiles_depth_kernel<KIND, NB, IL, OM, CONJ, ACC, WB, 3>), dim3(blocks),
+                           dim3(64 * WB), 0, s, a);
+        return;
+      }
+    } else {
+      if (a.depth == 2) {
+        hipLaunchKernelGGL((interp_tiles_depth_kernel<KIND, NB, IL, OM, CONJ, ACC, WB, 2>), dim3(blocks),
+                           dim3(64 * WB), 0, s, a);
+        return;
+      }
+    }
+  }
+  hipLaunchKernelGGL((interp_tiles_kernel<KIND, NB, IL, OM, CONJ, ACC, WB>), dim3(blocks), dim3(64 * WB), 0, s, a);
 }
 
 template <int KIND, int NB, bool IL, int OM, int WB = 1>
@@ -1034,6 +1081,7 @@ static int launch_interp_impl(const tsn_direction* d, int kind, const void* hi, 
     a.dc_row_stride = dc->row_stride;
     a.dc_coil_stride = dc->coil_stride;
   }
+  a.depth = d->pipe_depth;
   const uint32_t cw = d->coil_waves;
   if (kind == kSparseSplit) launch_tiles<kSparseSplit>(a, interleaved, out_mode, conj, accumulate, cw, stream);
   else if (kind == kDenseSplit) launch_tiles<kDenseSplit>(a, interleaved, out_mode, conj, accumulate, cw, stream);
@@ -1532,8 +1580,9 @@ static int upload_pack(const HostPack& p, tsn_direction** out) {
 constexpr uint32_t kMemberGroup = 8;  // default member group (batch autotuner candidate)
 
 int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* out, uint32_t member_group,
-                       uint32_t coil_waves) {
+                       uint32_t coil_waves, uint32_t depth) {
   *out = InterpBatch{};
+  if (depth > 3) return fail(TSN_ERR_ARGUMENT, "pipeline depth must be 0 (default) or 1..3");
   if (member_group == 0) member_group = kMemberGroup;
   if (coil_waves != 1 && coil_waves != 2) return fail(TSN_ERR_ARGUMENT, "coil_waves must be 1 or 2");
   if (parts.empty()) return fail(TSN_ERR_ARGUMENT, "empty interpolation batch");
@@ -1624,6 +1673,7 @@ int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* o
   InterpBatch b;
   b.blocks = blocks;
   b.coil_waves = coil_waves;
+  b.depth = depth;
   b.member_group = member_group;
   b.values = values;
   b.dense = std::all_of(parts.begin(), parts.end(), [](const InterpBatchPart& p) { return p.dir->tile_dense; });
@@ -1696,6 +1746,7 @@ int launch_interp_batch(const InterpBatch& b, int kind, int out_mode, bool conj,
   a.problems = static_cast<const InterpProblem*>(b.problems);
   a.item_problem = b.item_problem;
   a.coils = kCoilsPerBlock;  // replaced per problem
+  a.depth = b.depth;
   switch (kind) {
     case kSparseSplit: launch_batch_kind<kSparseSplit>(a, b.blocks, b.coil_waves, out_mode, conj, s); break;
     case kDenseSplit: launch_batch_kind<kDenseSplit>(a, b.blocks, b.coil_waves, out_mode, conj, s); break;
@@ -1810,6 +1861,15 @@ int tsn_direction_set_coil_waves(tsn_direction* d, uint32_t waves) {
 
 uint32_t tsn_direction_coil_waves(const tsn_direction* d) { return d ? d->coil_waves : 0; }
 
+int tsn_direction_set_pipe_depth(tsn_direction* d, uint32_t depth) {
+  if (!d) return fail(TSN_ERR_ARGUMENT, "null direction");
+  if (depth > 3) return fail(TSN_ERR_ARGUMENT, "pipeline depth must be 0 (default) or 1..3");
+  d->pipe_depth = depth;
+  return TSN_OK;
+}
+
+uint32_t tsn_direction_pipe_depth(const tsn_direction* d) { return d ? d->pipe_depth : 0; }
+
 int tsn_direction_autotune(tsn_direction* d, uint32_t coils, void* stream, uint32_t* best_unit_tiles,
                            float* best_us) {
   if (!d) return fail(TSN_ERR_ARGUMENT, "null direction");
@@ -1836,24 +1896,34 @@ int tsn_direction_autotune(tsn_direction* d, uint32_t coils, void* stream, uint3
   hip(hipEventCreate(&e0), "event");
   hip(hipEventCreate(&e1), "event");
   const int kind = d->values == TSN_VALUES_BF16 ? kSparseSplitBF16 : kSparseSplit;
-  const uint32_t candidates[] = {2, 3, 4, 5, 6, 8, 10, 12, 14, 16};
-  // 9..16 coils: one wave for both coil blocks, or one wave per block
-  std::vector<uint32_t> waves = {d->coil_waves};
-  if (coils > static_cast<uint32_t>(kCoilsPerBlock)) waves = {1u, 2u};
+  // candidates: unit sizes x, for 9..16 coils, one wave for both coil blocks
+  // or one per block x pipeline depths (the default and the alternatives of
+  // the launch: 1 or 3 tiles ahead for one block per wave, 2 for two)
+  struct Candidate {
+    uint32_t waves, unit, depth;
+  };
+  std::vector<Candidate> cands;
+  const bool two = coils > static_cast<uint32_t>(kCoilsPerBlock);
+  for (uint32_t cw : two ? std::vector<uint32_t>{1u, 2u} : std::vector<uint32_t>{d->coil_waves})
+    for (uint32_t u : {2u, 3u, 4u, 5u, 6u, 8u, 10u, 12u, 14u, 16u}) {
+      std::vector<uint32_t> depths = {0u};
+      if (two && cw == 1) depths.push_back(2u);
+      if (!two) depths.insert(depths.end(), {1u, 3u});
+      for (uint32_t dp : depths) cands.push_back({cw, u, dp});
+    }
   // two passes over the candidates, each candidate's faster pass counts
   // (neighbouring unit sizes differ by a few percent, about the spread of
   // one timing)
-  constexpr int kCandidates = sizeof(candidates) / sizeof(candidates[0]);
-  float times[2][kCandidates] = {};
-  uint32_t best = 0, best_waves = d->coil_waves;
+  std::vector<float> times(cands.size(), 0.0f);
+  Candidate best{d->coil_waves, 0, d->pipe_depth};
   float best_ms = 0.0f;
   for (int pass = 0; pass < 2; ++pass)
-  for (size_t wi = 0; wi < waves.size(); ++wi)
-  for (int ci = 0; ci < kCandidates; ++ci) {
+  for (size_t ci = 0; ci < cands.size(); ++ci) {
     if (rc != TSN_OK) break;
-    const uint32_t cw = waves[wi], u = candidates[ci];
-    d->coil_waves = cw;
-    rc = schedule_units(d, u);
+    const Candidate& c = cands[ci];
+    d->coil_waves = c.waves;
+    d->pipe_depth = c.depth;
+    rc = schedule_units(d, c.unit);
     auto launch = [&]() {
       return launch_interp(d, kind, panel, nullptr, coils, kOutSplit16IL, false, false, 1.0f, nullptr, nullptr,
                            nullptr, s, true, out);
@@ -1869,11 +1939,10 @@ int tsn_direction_autotune(tsn_direction* d, uint32_t coils, void* stream, uint3
     hip(hipEventElapsedTime(&ms, e0, e1), "event");
     ms /= kReps;
     if (rc != TSN_OK) break;
-    if (pass == 1) ms = std::min(ms, times[wi][ci]);
-    times[wi][ci] = ms;
-    if (pass == 1 && (best == 0 || ms < best_ms)) {
-      best = u;
-      best_waves = cw;
+    if (pass == 1) ms = std::min(ms, times[ci]);
+    times[ci] = ms;
+    if (pass == 1 && (best.unit == 0 || ms < best_ms)) {
+      best = c;
       best_ms = ms;
     }
   }
@@ -1882,10 +1951,11 @@ int tsn_direction_autotune(tsn_direction* d, uint32_t coils, void* stream, uint3
   if (e1) (void)hipEventDestroy(e1);
   if (panel) (void)hipFree(panel);
   if (out) (void)hipFree(out);
-  d->coil_waves = best_waves;
-  if (rc == TSN_OK) rc = schedule_units(d, best);
+  d->coil_waves = best.waves;
+  d->pipe_depth = best.depth;
+  if (rc == TSN_OK) rc = schedule_units(d, best.unit);
   if (rc != TSN_OK) return rc;
-  if (best_unit_tiles) *best_unit_tiles = best;
+  if (best_unit_tiles) *best_unit_tiles = best.unit;
   if (best_us) *best_us = best_ms * 1000.0f;
   return TSN_OK;
 }

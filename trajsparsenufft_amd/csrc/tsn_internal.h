@@ -207,14 +207,16 @@ struct InterpBatch {
   uint32_t n_problems = 0, n_items = 0, n_chunks = 0;
   uint32_t blocks = 1;  // coil blocks per part (every part of a batch has the same)
   uint32_t coil_waves = kDefaultCoilWaves;  // parts of two blocks: 1 wave for both, or 2 (one per block)
+  uint32_t depth = 0;                       // tiles in flight ahead of the multiply (0: the kernel's default)
   uint32_t member_group = 0;                // parts sharing a direction that follow each entry
   int values = TSN_VALUES_FP16;
   bool dense = false;
 };
 // member_group: parts sharing a direction that follow each of its entries
-// (0 = the default, 8); coil_waves: waves per entry of two-block parts.
+// (0 = the default, 8); coil_waves: waves per entry of two-block parts;
+// depth: tiles in flight ahead of the multiply (0 = the kernel's default).
 int interp_batch_build(const std::vector<InterpBatchPart>& parts, InterpBatch* out, uint32_t member_group = 0,
-                       uint32_t coil_waves = kDefaultCoilWaves);
+                       uint32_t coil_waves = kDefaultCoilWaves, uint32_t depth = 0);
 void interp_batch_free(InterpBatch* b);
 // kind: 0 split FP16, 1 dense control, 2 FP16 hi only, 3 split BF16; out_mode:
 // 1 row-major float2, 3 interleaved split (FP16 or BF16 as the kind).
@@ -293,6 +295,7 @@ struct tsn_direction {
   uint32_t n_slots = 0, n_counters = 0;  // sizes of the chunk scratch (ChunkScratch)
   uint32_t unit_tiles = 0;           // tile budget of one regular unit
   uint32_t coil_waves = tsn::kDefaultCoilWaves;  // interleaved launches of 9..16 coils: waves per entry (1 or 2)
+  uint32_t pipe_depth = 0;  // tiles in flight ahead of the multiply: 0 = the kernel's default, 1..3
   std::vector<uint32_t> host_offsets;  // group_offsets kept for rescheduling
   uint32_t* order = nullptr;         // [n_items + n_chunks] launch order (kChunkTag marks chunks)
   std::vector<tsn::WorkItem> host_items;  // the current schedule (batched launches merge it)

@@ -92,8 +92,9 @@ class Direction:
         return int(lib().tsn_direction_unit_tiles(self._h))
 
     def autotune(self, coils: int = 8) -> tuple[int, float]:
-        """Time the interpolation over the candidate unit sizes and keep the
-        fastest; returns (unit_tiles, microseconds per launch)."""
+        """Time the interpolation over the candidate unit sizes, pipeline depths
+        and (9..16 coils) waves per entry, and keep the fastest; returns
+        (unit_tiles, microseconds per launch)."""
 
         best, us = C.c_uint32(), C.c_float()
         check(lib().tsn_direction_autotune(self._h, int(coils), _stream(), C.byref(best), C.byref(us)))
@@ -108,6 +109,16 @@ class Direction:
     @coil_waves.setter
     def coil_waves(self, waves: int) -> None:
         check(lib().tsn_direction_set_coil_waves(self._h, int(waves)))
+
+    @property
+    def pipe_depth(self) -> int:
+        """Tiles in flight ahead of the multiply in interleaved-panel launches
+        (0: the kernel's default; 1..3 where the launch offers it)."""
+        return int(lib().tsn_direction_pipe_depth(self._h))
+
+    @pipe_depth.setter
+    def pipe_depth(self, depth: int) -> None:
+        check(lib().tsn_direction_set_pipe_depth(self._h, int(depth)))
 
     def schedule(self, unit_tiles: int = 0) -> None:
         """Re-cut the work units (runs of whole groups of at most unit_tiles

@@ -250,12 +250,15 @@ class Batch:
 
     def autotune(self) -> dict:
         """Time the batched interpolation steps over the member group (frames
-        sharing a pack per work entry) and the waves per entry of 9..16-coil
-        problems, and keep the fastest (results unchanged bit for bit)."""
+        sharing a pack per work entry), the pipeline depth and the waves per
+        entry of 9..16-coil problems, and keep the fastest (results unchanged
+        bit for bit)."""
 
-        group, waves, us = C.c_uint32(), C.c_uint32(), C.c_float()
-        check(lib().tsn_batch_autotune(self._h, _stream(), C.byref(group), C.byref(waves), C.byref(us)))
-        return {"member_group": int(group.value), "coil_waves": int(waves.value), "us": float(us.value)}
+        group, waves, depth, us = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_float()
+        check(lib().tsn_batch_autotune(self._h, _stream(), C.byref(group), C.byref(waves), C.byref(depth),
+                                       C.byref(us)))
+        return {"member_group": int(group.value), "coil_waves": int(waves.value), "pipe_depth": int(depth.value),
+                "us": float(us.value)}
 
     def cg(self, measurements, iterations: int = 10, operator_scales=None, lam: float = 1e-4, outs=None):
         """Enqueue all solves on the current stream; returns one CgResult per problem."""
