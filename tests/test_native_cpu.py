@@ -177,3 +177,20 @@ def test_dense_expansion_of_native_pack_round_trips():
     dense = layout.expand_tiles(comp, meta)
     comp2, _ = layout.compress_tiles_vectorized(dense.astype(np.float64))
     np.testing.assert_array_equal(comp2.view(np.uint16), comp.view(np.uint16))
+
+
+def test_build_flags_kernels_with_scratch_memory():
+    """The build parses the compiler's per-kernel resource remarks and names
+    every kernel with private (scratch) memory, which it then refuses."""
+
+    from trajsparsenufft_amd.build import scratch_kernels
+    remarks = "\n".join([
+        "a.hip:10:1: remark: Function Name: _Z4fastv [-Rpass-analysis=kernel-resource-usage]",
+        "a.hip:10:1: remark:     VGPRs: 40 [-Rpass-analysis=kernel-resource-usage]",
+        "a.hip:10:1: remark:     ScratchSize [bytes/lane]: 0 [-Rpass-analysis=kernel-resource-usage]",
+        "a.hip:20:1: remark: Function Name: _Z5spillv [-Rpass-analysis=kernel-resource-usage]",
+        "   20 | __global__ void spill() {",
+        "a.hip:20:1: remark:     ScratchSize [bytes/lane]: 48 [-Rpass-analysis=kernel-resource-usage]",
+    ])
+    assert scratch_kernels(remarks) == ["_Z5spillv"]
+    assert scratch_kernels("") == []

@@ -47,19 +47,46 @@ def _digest(paths, flags) -> str:
     return h.hexdigest()[:16]
 
 
+RESOURCE_REMARKS = ["-Rpass-analysis=kernel-resource-usage"]
+
+
+def scratch_kernels(remarks: str) -> list[str]:
+    """Kernels whose resource-usage remarks report private (scratch) memory:
+    spilled registers or a local copy of a by-value argument, each a round
+    trip to memory per access in a hot loop."""
+
+    found, name = [], None
+    for line in remarks.splitlines():
+        if "remark: Function Name: " in line:
+            name = line.split("Function Name: ", 1)[1].split(" [-R")[0].strip()
+        elif "ScratchSize [bytes/lane]: " in line and name:
+            if int(line.split("ScratchSize [bytes/lane]: ", 1)[1].split()[0]) > 0:
+                found.append(name)
+    return found
+
+
 def _compile(source: str, force: bool) -> Path:
     src = CSRC / source
     headers = sorted(CSRC.glob("*.h")) + sorted(INCLUDE.glob("*.h"))
-    flags = COMMON + DEVICE + (["-x", "hip"] if source.endswith(".hip") else [])
+    hip_src = source.endswith(".hip")
+    flags = COMMON + DEVICE + (["-x", "hip"] + RESOURCE_REMARKS if hip_src else [])
     obj = BUILD / f"{src.stem}-{_digest([src, *headers], flags)}.o"
     if obj.exists() and not force:
         return obj
     cmd = [hipcc(), *flags, "-c", str(src), "-o", str(obj) + ".tmp"]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     noise = "'-packed-fp32-ops' is not a recognized feature"
-    log = "\n".join(l for l in (proc.stdout + proc.stderr).splitlines() if noise not in l)
+    out = proc.stdout + proc.stderr
+    # resource remarks (and the source excerpts printed under them) are parsed, not shown
+    lines = [l for l in out.splitlines() if noise not in l]
+    log = "\n".join(l for l in lines if "kernel-resource-usage" not in l and not l.startswith(" ")
+                    and not l.startswith("In file included from"))
     if proc.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {source}:\n{log}")
+        full = "\n".join(l for l in lines if "kernel-resource-usage" not in l)
+        raise RuntimeError(f"hipcc failed for {source}:\n{full}")
+    spills = scratch_kernels(out) if hip_src else []
+    if spills:
+        raise RuntimeError(f"{source}: kernels use scratch memory (spills or argument copies): {spills}")
     if log.strip():
         print(log, file=sys.stderr)
     os.replace(str(obj) + ".tmp", obj)
