@@ -655,6 +655,7 @@ struct CsrArgs {
   uint32_t n_out;
   float scale;
   float2* out;
+  uint32_t skip_empty;  // the output's empty rows already hold zeros: runs of empty rows write nothing
 };
 
 // Plain FP32 path.  One wave per block takes one work item: a run of at most
@@ -757,6 +758,20 @@ __global__ __launch_bounds__(64) void interp_csr_kernel(CsrArgs a) {
   const uint32_t g = lane / LPR, c = lane % LPR;
   const bool active = c < a.coils;
   const bool chunk = it.n_chunks != 0;
+  if (n == 0) {
+    // a run of empty rows (at most kCsrEmptyRunRows): zeros, unless the
+    // output holds them already or the launch accumulates; every row id is
+    // loaded before the first store
+    if (ACC || a.skip_empty) return;
+    constexpr int kPer = (kCsrEmptyRunRows + G - 1) / G;
+    int32_t orow[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) orow[k] = glb(a.row_ids)[min(it.row_begin + g + k * G, it.row_end - 1)];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+      if (it.row_begin + g + k * G < it.row_end && active) csr_store<OM, false>(a, orow[k], c, 0.f, 0.f);
+    return;
+  }
   // the group's row of a run (its first; runs hold at most G rows when LPR <= 8)
   const uint32_t row = chunk ? it.row_begin : min(it.row_begin + g, it.row_end - 1);
   const uint32_t rb = glb(a.row_ptr)[row], re_ = glb(a.row_ptr)[row + 1];
@@ -1175,7 +1190,7 @@ static void launch_csr_lpr(const CsrArgs& a, int om, bool conj, bool acc, hipStr
 
 int launch_interp_csr(const tsn_direction* d, const float2* in, uint32_t coils, int out_mode,
                       bool conj, bool accumulate, float scale, float2* out, hipStream_t stream,
-                      const ChunkScratch* scratch) {
+                      const ChunkScratch* scratch, bool out_prezeroed) {
   if (!d || !d->csr) return fail(TSN_ERR_ARGUMENT, "FP32 interpolation needs a CSR direction");
   if (coils == 0 || coils > static_cast<uint32_t>(kMaxCoils))
     return fail(TSN_ERR_ARGUMENT, "coils must be in 1..16 (got %u)", coils);
@@ -1201,6 +1216,7 @@ int launch_interp_csr(const tsn_direction* d, const float2* in, uint32_t coils, 
   a.n_out = d->h.output_size;
   a.scale = scale;
   a.out = out;
+  a.skip_empty = out_prezeroed ? 1u : 0u;
   if (a.n_items == 0) return TSN_OK;
   if (coils <= 4) launch_csr_lpr<4>(a, out_mode, conj, accumulate, stream);
   else if (coils <= 8) launch_csr_lpr<8>(a, out_mode, conj, accumulate, stream);
@@ -1407,9 +1423,13 @@ std::vector<tsn::CsrItem> tsn::csr_items(const std::vector<uint32_t>& ptr, uint3
       continue;
     }
     const uint32_t start = t;
-    while (t < rows && t - start < static_cast<uint32_t>(kCsrRowsPerItem) &&
-           ptr[t + 1] - ptr[t] <= static_cast<uint32_t>(kCsrShortEntries))
-      ++t;
+    if (len == 0) {  // a run of empty rows
+      while (t < rows && t - start < static_cast<uint32_t>(kCsrEmptyRunRows) && ptr[t + 1] == ptr[t]) ++t;
+    } else {  // empty rows after the first stay in the run (cheap there)
+      while (t < rows && t - start < static_cast<uint32_t>(kCsrRowsPerItem) &&
+             ptr[t + 1] - ptr[t] <= static_cast<uint32_t>(kCsrShortEntries))
+        ++t;
+    }
     items.push_back(CsrItem{start, t, ptr[start], ptr[t], 0, 0, 0, 0});
   }
   return items;

@@ -154,7 +154,7 @@ int op_forward_samples(tsn_plan* pl, float2* out_coil_major, hipStream_t s) {
 int op_adjoint_grid(tsn_plan* pl, hipStream_t s) {
   if (is_fp32(pl))
     return launch_interp_csr(pl->adj, pl->smp32, pl->cfg.coils, 1, true, false, 1.0f, pl->grid, s,
-                             &pl->adj_chunks);
+                             &pl->adj_chunks, true);
   // row-major (mode 1) up to 8 coils, 8-coil blocks of rows (mode 5) above
   return launch_interp(pl->adj, interp_kind(pl), pl->smp_il, nullptr, pl->cfg.coils,
                        pl->cfg.coils > static_cast<uint32_t>(kCoilsPerBlock) ? 5 : 1, true, false, 1.0f, reinterpret_cast<float*>(pl->grid),

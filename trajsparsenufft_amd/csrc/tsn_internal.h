@@ -25,9 +25,13 @@ constexpr int kSplitAboveTiles = 16;  // tiles per unit at most; longer groups a
 // wave computes both coil blocks) or 2 (one wave per block, same entry).
 constexpr uint32_t kDefaultCoilWaves = 1;
 // FP32 CSR work items: runs of at most kCsrRowsPerItem consecutive rows of at
-// most kCsrShortEntries entries each; longer rows are cut into chunks of at
-// most kCsrChunkEntries entries.  A wave stages an item's entries in LDS.
+// most kCsrShortEntries entries each, the first not empty; longer rows are
+// cut into chunks of at most kCsrChunkEntries entries; a run that would start
+// with an empty row takes the consecutive empty rows instead, at most
+// kCsrEmptyRunRows (no entries to stage).  A wave stages an item's entries in
+// LDS.
 constexpr int kCsrRowsPerItem = 8;
+constexpr int kCsrEmptyRunRows = 64;
 constexpr int kCsrShortEntries = 64;
 constexpr int kCsrChunkEntries = 256;
 constexpr int kCsrStageEntries = kCsrRowsPerItem * kCsrShortEntries;
@@ -232,9 +236,11 @@ struct ChunkScratch {
   uint32_t* counters = nullptr;
 };
 // FP32 CSR interpolation; scratch null: the direction's own chunk scratch.
+// out_prezeroed: the output's empty rows hold zeros already (a plan's grid,
+// zeroed at creation and written only here), so they are not written again.
 int launch_interp_csr(const tsn_direction* d, const float2* in, uint32_t coils, int out_mode, bool conj,
                       bool accumulate, float scale, float2* out, hipStream_t stream,
-                      const ChunkScratch* scratch = nullptr);
+                      const ChunkScratch* scratch = nullptr, bool out_prezeroed = false);
 // Packed interpolation (kind and out_mode as below); scratch null: the
 // direction's own.
 int launch_interp(const tsn_direction* d, int kind, const void* hi, const void* res, uint32_t coils,
