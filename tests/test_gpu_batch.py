@@ -42,7 +42,9 @@ def _bits(t):
 @pytest.mark.parametrize("precision", ["split16", "splitbf16", "dense16"])
 def test_uneven_slices_match_single_plans(precision):
     from trajsparsenufft_amd.recon import build_batch
-    specs = [(32, 16, 4, 1), (32, 40, 12, 2), (32, 24, 8, 3), (32, 30, 16, 4)]
+    # coil counts of every channel layout: compact rows (4, 8, odd 5), two
+    # 8-coil blocks (12, 16, odd 11)
+    specs = [(32, 16, 4, 1), (32, 40, 12, 2), (32, 24, 8, 3), (32, 30, 16, 4), (32, 20, 5, 5), (32, 28, 11, 6)]
     cases = [_problem(*s) for s in specs]
     singles, scales = [], []
     for packs, smaps, y in cases:
