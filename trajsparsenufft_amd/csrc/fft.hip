@@ -257,7 +257,8 @@ __global__ __launch_bounds__(512) void inv_axis1_kernel(FftArgs args) {
   constexpr uint32_t kRowsPerBlock = rows_per_block<R>();
   const uint32_t row0 = xcd_remap(blockIdx.x, gridDim.x) * kRowsPerBlock, chunk = blockIdx.y;
   const uint32_t cw = min(8u, a.coils - chunk * 8);
-  const ChunkRows gl = a.grid_blocked ? chunk_rows(a.coils, static_cast<uint64_t>(a.k0) * K) : ChunkRows{a.cstride, 8};
+  const ChunkRows gl =
+      a.grid_blocked ? chunk_rows(a.coils, static_cast<uint64_t>(a.k0) * K) : ChunkRows{a.cstride, 8};
   const ChunkRows tl = chunk_rows(a.coils, static_cast<uint64_t>(a.n1) * a.k0);
   auto g_row = [&](uint32_t k0) { return a.grid + chunk * gl.cs + static_cast<size_t>(k0) * K * gl.rs; };
   ExchangeRegs<K> rows;

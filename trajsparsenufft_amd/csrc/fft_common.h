@@ -12,7 +12,8 @@ struct FftArgs {
   uint32_t n0, n1;         // image shape
   uint32_t k0, k1;         // grid shape
   uint32_t coils;          // coils of the problem (per problem: coils_of[z] when set)
-  uint32_t cstride;        // channel stride of the [.][coil] layouts (>= every problem's coils)
+  uint32_t cstride;        // >= every problem's coils: SENSE-map channel stride, coil chunks of the
+                           // launch, row width of the legacy grid layout
   const float2* tw0;       // W_K0^k, k < K0
   const float2* tw1;       // W_K1^k
   const float2* image_t;   // [n1][n0]
