@@ -198,8 +198,11 @@ void tsn_direction_free(tsn_direction* dir);
  * of FP16 N16 panels ([row][16] halves: Re coils 0..7, Im coils 0..7, blocks
  * of 8 coils, block stride n_cols*16 halves).  panel_res may be NULL (hi
  * only).  Output float2 per layout; accumulate adds to it.  scale multiplies
- * the result.  Equivalent to launch_fused_interpolation followed by
- * launch_fused_interpolation_residual (nufft_integrated_bench.cu:431-466). */
+ * the result.  Only the rows of the pack's groups are written; other output
+ * rows (grid points no sample reaches, for G^T) keep their contents, as in
+ * the reference, whose harness zeroes the grid once.  Equivalent to
+ * launch_fused_interpolation followed by launch_fused_interpolation_residual
+ * (nufft_integrated_bench.cu:431-466). */
 int tsn_interp(const tsn_direction* dir, const void* panel_hi, const void* panel_res, void* out,
                uint32_t coils, int conjugate, int accumulate, float scale, int out_layout,
                void* stream);
@@ -221,7 +224,10 @@ int tsn_interp_noalloc(const tsn_direction* dir, const void* panel_hi, const voi
 int tsn_interp_dense(const tsn_direction* dir, const void* panel_hi, const void* panel_res,
                      void* out, uint32_t coils, int conjugate, int accumulate, float scale,
                      int out_layout, void* stream);
-/* Plain FP32 path: CSR direction, input float2 [row][coils], output per layout. */
+/* Plain FP32 path: CSR direction, input float2 [row][coils], output per layout.
+ * Without accumulate every output row is written, zeros where no entry
+ * reaches, like the full sparse product of the reference's FP32 path
+ * (torch.sparse.mm, experiments/production/reference_sparse_solver.py:118-129). */
 int tsn_interp_f32(const tsn_direction* csr, const void* in, void* out, uint32_t coils,
                    int conjugate, int accumulate, float scale, int out_layout, void* stream);
 /* hi = RN16(v), res = RN16(v - hi) of float2 [coil][rows] * scale into N16 panel

@@ -116,6 +116,34 @@ def test_fp32_csr_matches_oracle(view, coils, spokes):
     assert np.array_equal(again.cpu().numpy().view(np.uint32), rm.cpu().numpy().view(np.uint32))
 
 
+@pytest.mark.parametrize("coils", [8, 3])
+@pytest.mark.parametrize("layout", ["coil_major", "row_major"])
+def test_fp32_csr_writes_every_row_over_garbage(coils, layout):
+    """G_T of a sparse radial trajectory has empty rows (grid points no
+    sample reaches), which the FP32 kernel covers with runs of empty rows:
+    an output first filled with NaN comes back finite, zero on the empty
+    rows and equal to the float64 product elsewhere."""
+
+    from oracle.solver import apply_csr_adjoint
+    from trajsparsenufft_amd import device
+    _, packs = small_case(32, 12)
+    cols, re, im = packs.kb.arrays()
+    d = device.Direction.csr(packs.kb, "G_T")
+    counts = np.bincount(cols.ravel(), minlength=d.rows)
+    assert (counts == 0).sum() > 64  # runs of empty rows exist
+    rng = np.random.default_rng(coils + 11)
+    x = (rng.standard_normal((d.cols, coils)) + 1j * rng.standard_normal((d.cols, coils))).astype(np.complex64)
+    ref = apply_csr_adjoint(cols, re, im, x.astype(np.complex128), d.rows).T  # [coil][row]
+    shape = (coils, d.rows) if layout == "coil_major" else (d.rows, coils)
+    out = torch.full(shape, float("nan"), dtype=torch.complex64, device="cuda")
+    device.interp_f32(d, dev(x), out, coils=coils, conjugate=True, scale=1.0, layout=layout)
+    got = out.cpu().numpy()
+    got = got if layout == "coil_major" else np.ascontiguousarray(got.T)
+    assert np.isfinite(got.view(np.float32)).all()
+    assert np.array_equal(got[:, counts == 0], np.zeros_like(got[:, counts == 0]))
+    assert rel_l2(got, ref) <= 1e-6
+
+
 @pytest.mark.parametrize("coils,rows", [(12, 1000), (5, 130), (16, 64), (3, 1)])
 def test_split_panel_is_bitwise_numpy_rounding(coils, rows):
     """Both split kernels (separate hi/residual panels and the interleaved

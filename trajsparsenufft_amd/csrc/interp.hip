@@ -1217,6 +1217,7 @@ int launch_interp_csr(const tsn_direction* d, const float2* in, uint32_t coils, 
   a.scale = scale;
   a.out = out;
   a.skip_empty = out_prezeroed ? 1u : 0u;
+  if (out_prezeroed || accumulate) a.n_items = d->n_csr_active_items;  // the trailing empty runs write nothing
   if (a.n_items == 0) return TSN_OK;
   if (coils <= 4) launch_csr_lpr<4>(a, out_mode, conj, accumulate, stream);
   else if (coils <= 8) launch_csr_lpr<8>(a, out_mode, conj, accumulate, stream);
@@ -1399,7 +1400,8 @@ static int schedule_units(tsn_direction* d, uint32_t unit_tiles) {
 // most kCsrShortEntries entries (at most kCsrRowsPerItem rows each), and each
 // longer row cut into ceil(entries / kCsrChunkEntries) chunks of equal size
 // (within one entry); a row of several chunks takes that many partial slots
-// and one arrival counter.  Every CSR row has at least one entry.
+// and one arrival counter.  Empty rows form runs of their own (at most
+// kCsrEmptyRunRows rows) unless they follow a short row in a run.
 std::vector<tsn::CsrItem> tsn::csr_items(const std::vector<uint32_t>& ptr, uint32_t* slots, uint32_t* counters) {
   std::vector<CsrItem> items;
   *slots = 0;
@@ -2080,8 +2082,24 @@ int tsn_direction_upload_csr(const tsn_kb_matrix* kb, int view, tsn_direction** 
     cols.swap(ncols);
     vals.swap(nvals);
   }
+  // Output rows no entry reaches (grid points of the transpose no sample
+  // touches) follow as empty CSR rows in ascending order, so a launch that
+  // does not accumulate writes zeros there too (runs of empty rows).  Their
+  // items come last; a launch into an output already zero leaves them out.
+  const uint32_t active = static_cast<uint32_t>(rows.size());
+  if (active < h.output_size) {
+    std::vector<uint8_t> hit(h.output_size, 0);
+    for (int32_t r : rows) hit[r] = 1;
+    for (uint32_t r = 0; r < h.output_size; ++r)
+      if (!hit[r]) {
+        rows.push_back(static_cast<int32_t>(r));
+        ptr.push_back(ptr.back());
+      }
+  }
   uint32_t slots = 0, counters = 0;
   const std::vector<CsrItem> items = csr_items(ptr, &slots, &counters);
+  uint32_t active_items = 0;  // items holding an active row: all but the trailing runs of empty rows
+  while (active_items < items.size() && items[active_items].row_begin < active) ++active_items;
   if (slots > kMaxCsrSlots) return fail(TSN_ERR_ARGUMENT, "%u long-row chunks (at most %u)", slots, kMaxCsrSlots);
   auto* d = new tsn_direction();
   d->h = h;
@@ -2089,6 +2107,7 @@ int tsn_direction_upload_csr(const tsn_kb_matrix* kb, int view, tsn_direction** 
   (void)hipGetDevice(&d->device);
   d->csr_rows = static_cast<uint32_t>(rows.size());
   d->n_csr_items = static_cast<uint32_t>(items.size());
+  d->n_csr_active_items = active_items;
   d->n_slots = slots;
   d->n_counters = counters;
   size_t bytes = 0;

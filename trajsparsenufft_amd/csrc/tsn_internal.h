@@ -315,6 +315,7 @@ struct tsn_direction {
   float2* vals = nullptr;        // [nnz]
   tsn::CsrItem* csr_items = nullptr;  // [n_csr_items] work items in launch order
   uint32_t n_csr_items = 0;
+  uint32_t n_csr_active_items = 0;  // leading items holding a row with entries (the rest zero empty rows)
   // (partials / counters / n_slots / n_counters above hold the long-row chunk
   // scratch of a CSR direction: kMaxCoils float2 per slot)
   size_t bytes = 0;
