@@ -144,6 +144,33 @@ def test_fp32_csr_writes_every_row_over_garbage(coils, layout):
     assert rel_l2(got, ref) <= 1e-6
 
 
+def test_packed_adjoint_writes_only_group_rows():
+    """The packed G^T writes the rows of the pack's groups (the grid points
+    some sample reaches) and leaves every other output row as it was, the
+    pack format's convention (plans zero their grid once)."""
+
+    from oracle.solver import apply_pack
+    from trajsparsenufft_amd import device
+    coils = 8
+    _, packs = small_case(32, 12)
+    pack = packs.adjoint
+    arrays, header = pack.arrays(), pack.header
+    d = device.Direction.upload(pack)
+    n_out = header["output_size"]
+    reached = np.zeros(n_out, dtype=bool)
+    rows = np.asarray(arrays["group_row_ids"])
+    reached[rows[rows >= 0]] = True
+    assert 64 < (~reached).sum() < n_out  # a sparse trajectory leaves grid points unreached
+    _, hi, res = random_panels(header["n_cols"], coils, seed=21)
+    sentinel = np.complex64(3.0 - 7.0j)
+    out = torch.full((coils, n_out), complex(sentinel), dtype=torch.complex64, device="cuda")
+    device.interp(d, dev(hi), dev(res), out, coils=coils, conjugate=True)
+    got = out.cpu().numpy()
+    assert np.array_equal(got[:, ~reached], np.full_like(got[:, ~reached], sentinel))
+    ref = apply_pack(arrays, n_out, hi, res, coils, True)
+    assert rel_l2(got[:, reached], ref[:, reached]) <= TOL
+
+
 @pytest.mark.parametrize("coils,rows", [(12, 1000), (5, 130), (16, 64), (3, 1)])
 def test_split_panel_is_bitwise_numpy_rounding(coils, rows):
     """Both split kernels (separate hi/residual panels and the interleaved
