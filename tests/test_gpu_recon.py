@@ -44,14 +44,15 @@ def _model(packs, smaps, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "split16", "dense16", "hi16", "splitbf16"])
-@pytest.mark.parametrize("coils", [8, 4, 12, 3, 16, 11])
+@pytest.mark.parametrize("coils", [8, 4, 12, 3, 16, 11, 1, 9])
 def test_forward_adjoint_normal_match_model(precision, coils):
     """Plan operators vs the arithmetic model with the same roundings.  BF16
     keeps 8 mantissa bits per half, so an FP32 FFT value next to a BF16
     rounding boundary splits differently from the float64 model's: 1e-4.
     The coil counts cover every channel layout of the FFT intermediates and
-    grids: compact rows of an even and an odd coil count (4, 8, 3), and two
-    8-coil blocks of 64-byte rows, full or partly empty (16, 12, 11)."""
+    grids: compact rows of an even and an odd coil count (4, 8, 3), a single
+    coil (1), and two 8-coil blocks of 64-byte rows, full or partly empty
+    (16, 12, 11, and 9: a second block of one coil)."""
 
     tol = 1e-4 if precision == "splitbf16" else 2e-5
     _, packs = small_case()
