@@ -116,7 +116,7 @@ def test_fp32_csr_matches_oracle(view, coils, spokes):
     assert np.array_equal(again.cpu().numpy().view(np.uint32), rm.cpu().numpy().view(np.uint32))
 
 
-@pytest.mark.parametrize("coils", [8, 3])
+@pytest.mark.parametrize("coils", [8, 3, 16])
 @pytest.mark.parametrize("layout", ["coil_major", "row_major"])
 def test_fp32_csr_writes_every_row_over_garbage(coils, layout):
     """G_T of a sparse radial trajectory has empty rows (grid points no
